@@ -1,0 +1,216 @@
+/*
+ * mbtrack-mi355x: C ABI of the MI355X multi-bunch tracking turn.
+ *
+ * One process drives one GPU.  Every bunch of the ring lives in HBM as
+ * structure-of-arrays FP64 coordinates; the per-bunch MPI ranks of the
+ * reference become thread blocks, and its per-turn MPI_Bcast "allgathers"
+ * become device arrays read by the next kernel.
+ *
+ * All functions return MBT_OK (0) or a negative MBT_E* code; mbt_last_error()
+ * returns a human-readable message for the calling thread's last failure.
+ * No torch or HIP types appear in these signatures.
+ *
+ * Reference interfaces replaced (fracul/mtrack, file:line):
+ *   mbt_config_read        read_input + setup_ring_parameters +
+ *                          macrop_model_setup_parameters + e_beam_setup +
+ *                          setup_tracking_parameters   src/input.c:136,1553,1775,1289,1698
+ *   mbt_generate_bunch     weak_generate_bunch_distribution src/bunch_weak.c:172-292
+ *   mbt_ring_create        worker_weakweak set-up   src/worker_weakweak.c:58-255
+ *                          (Green functions src/transform_weak.c:289-471,
+ *                          LR histograms + wake_phasor_init :839-952,
+ *                          RW history src/worker_weakweak.c:203-219)
+ *   mbt_ring_track         the worker turn loop src/worker_weakweak.c:259-394 for
+ *                          all bunches at once: transform_weak_bunch_selffield
+ *                          src/transform_weak.c:553-771, RF feedback
+ *                          src/worker_weakweak.c:268-278 + src/feedback_rf.c:5-34,
+ *                          transform_weak_bunch_optic src/transform_weak.c:21-192,
+ *                          weak_bunch_calc_statistics src/statistics.c:131-157,
+ *                          weak_bunch_calc_ampinv src/statistics.c:163-195,
+ *                          transform_weak_bunch_RW_longrange_cyclic
+ *                          src/transform_weak.c:780-833, fnp_ring_update +
+ *                          construct_wake_phasor src/transform_weak.c:839-1048
+ *   mbt_manager_weakweak   manager_weakweak + worker_weakweak as one process
+ *                          src/manager_weakweak.c:14-350, src/worker_weakweak.c:19-471
+ *   mbt_wofz               complex_errorfunc_wofz_algo680 src/physic.c:75-301
+ */
+#ifndef MBTRACK_MI355X_H
+#define MBTRACK_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBT_OK 0
+#define MBT_EINVAL -1  /* bad argument or configuration */
+#define MBT_EIO -2     /* file could not be read or written */
+#define MBT_EDEVICE -3 /* HIP runtime failure */
+#define MBT_ENOMEM -4
+#define MBT_EUNSUPPORTED -5
+
+#define MBT_LON 0
+#define MBT_HOR 1
+#define MBT_VER 2
+
+#define MBT_MAX_RES 16
+#define MBT_MAX_LR 8
+#define MBT_MAX_AHC 8
+#define MBT_MAX_WAKEFILE 8
+#define MBT_PATH 1024
+
+/* Filling patterns: numerical values of the reference's filling_t, src/def.h:31-47 */
+#define MBT_FILL_FROMFILE -1
+#define MBT_FILL_UNIFORM 0
+
+/* Stats layout: per bunch per turn 12 doubles in the order a worker sends them
+ * (src/worker_weakweak.c:321-324): pos[3] pos_sigma[3] slope[3] slope_sigma[3],
+ * each indexed LON, HOR, VER.  Ampinv: ampinv[3] ampinv_cm[3]. */
+#define MBT_NSTAT 12
+#define MBT_NAMP 6
+
+typedef struct mbt_resonator {
+  double Rs, wr, Qfactor;
+  int plane;
+} mbt_resonator;
+
+typedef struct mbt_lr_resonator {
+  double Rs, wr, Qfactor, detune;
+  int m;
+  unsigned Nturn, Nbu;
+} mbt_lr_resonator;
+
+typedef struct mbt_active_hc {
+  double Vpeak, nHC, phi_aHC;
+} mbt_active_hc;
+
+/* Every input of the .conf file and every quantity derived from it.  Field names
+ * are the reference's physics names (src/types.h:60-209, src/bunch.h:23-49). */
+typedef struct mbt_config {
+  /* [tracking] */
+  char jobtitle[256];
+  char input_filename[MBT_PATH];
+  char work_path[MBT_PATH];
+  char fill_filename[MBT_PATH];
+  int filling, bunch_model;
+  int TrackPlane[3];
+  long NrevTot, NrevMon, NrevPotentialsOut;
+  int EnableRW_short, EnableRW_short_LON, EnableRW_long, EnableFBII, EnableQuantum;
+  int EnableIdealHC, EnableAmpinv_out, EnableDiffCurr, EnableResonators;
+  double current_ratio;
+  int Nmlt, NmultiT, triggerRW;
+  int scan;
+  long Nscan, NrevScan;
+  double scan_start, scan_step;
+  double s0, mult;
+  /* [ring] inputs */
+  double E0, Lc, rho0, ac, Je, Vrf0, frf, Iring, ApertureH, ApertureV;
+  double QH0, QV0, CHH, CHV, CVH, CVV, Gzix, Gziz, emittanceH, couplbeta;
+  double rhorw, beffL[2], beffH[2], beffV[2], fYokoyaV, fYokoyaH;
+  double beta1[3], alpha1[3], gamma1[3], dispH1, disppH1;
+  double U0, taue;
+  /* [ring] derived (src/input.c:1553-1695) */
+  unsigned Nharm;
+  double h, R, Gamma, Gamma2, Gamma3, wrf, T0, w0, Urad, De, tauC, aexpe;
+  double q, Fq, epsMax, Vrfp, wso, wso2, fso, Qso, wgziH, wgziV, sn0, phai0, fc1, fc12;
+  double Jx, Jz, taux, tauz;
+  int AmpDependdQH, AmpDependdQV;
+  int m_aHC;
+  double phi_n, HC_k;
+  /* impedances */
+  int n_res;
+  mbt_resonator res[MBT_MAX_RES];
+  int n_wakefile;
+  char wakefile[MBT_MAX_WAKEFILE][MBT_PATH];
+  int wakefile_plane[MBT_MAX_WAKEFILE];
+  int n_lr[3];
+  mbt_lr_resonator lr[3][MBT_MAX_LR];
+  unsigned Nbumax;
+  int has_rf_feedback, rffb_resonator, rffb_len_average;
+  double rffb_vrf_design, rffb_phi0_design;
+  int n_ahc;
+  mbt_active_hc ahc[MBT_MAX_AHC];
+  /* [selffield] */
+  int Ncell;
+  double Nsigma, dT, sigma_tau;
+  double RsisL, aindL, RsisV, aindV, RsisH, aindH;
+  /* [distribution] */
+  int Nslc, Np;
+  double fNp;
+  int modeHT, mode_excitation;
+  int nGen[3], iseed[3];
+  char gendst_datafile[3][MBT_PATH];
+  double xtauCM_offset, xepsCM_offset, sgm_xtau, sgm_xeps, corr_epstau;
+  double xxCM_offset, xpCM_offset, sgm_xx, sgm_xp, corr_xpx;
+  double zzCM_offset, zpCM_offset, sgm_zz, sgm_zp, corr_zpz;
+  double sgmatau;
+  double pos_offset[3], slope_offset[3], pos_sgm[3], slope_sgm[3], correlate[3];
+  /* filling (src/input.c:1289-1512) and currents (src/manager_weakweak.c:58-93) */
+  int Nbunch;
+  int *nfFill;      /* [Nharm] */
+  double *Ib_frac;  /* [Nbunch], by filled-bunch ordinal */
+  double *Ibunch;   /* [Nharm] bunch current [mA] */
+  double *bunch_Ib; /* [Nbunch] bunch current [A] */
+  int *bucket;      /* [Nbunch] bucket of the k-th filled bunch */
+  int Nbunch_out;
+  int *bunch_out;
+} mbt_config;
+
+const char *mbt_last_error(void);
+
+/* ---- host configuration (C) ------------------------------------------------ */
+int mbt_config_read(const char *conf_path, const char *work_path, mbt_config *cfg);
+void mbt_config_free(mbt_config *cfg);
+/* Writes every scalar of the configuration as "name value" (%.17g), one per line. */
+int mbt_config_dump(const mbt_config *cfg, const char *path);
+/* Green functions Gl/Gv/Gh[Ncell] for the configuration (src/transform_weak.c:289-471);
+ * counts of contributing impedances per plane in planes[3] (PlaneL, PlaneH, PlaneV order LON,HOR,VER). */
+int mbt_greens_functions(const mbt_config *cfg, double *Gl, double *Gv, double *Gh, int planes[3]);
+/* Initial coordinates of bucket kb, SoA: out[0..Np) tau, [Np..2Np) delta, x, xp, z, zp. */
+int mbt_generate_bunch(const mbt_config *cfg, int kb, double *out);
+
+/* ---- the ring on the GPU ---------------------------------------------------- */
+typedef struct mbt_ring mbt_ring;
+
+/* flags */
+#define MBT_RING_NO_INIT_DIST 1 /* leave coordinates zero; caller uploads them */
+
+int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
+void mbt_ring_destroy(mbt_ring *r);
+/* Replace the coordinates of filled bunch ordinal ib (SoA layout as mbt_generate_bunch). */
+int mbt_ring_set_bunch(mbt_ring *r, int ib, const double *soa6);
+int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *soa6);
+/* Finishes set-up after coordinates are final: initial statistics, LR-resonator
+ * phasor initialisation, RF-feedback design solve and RW history fill.
+ * init_stats receives Nbunch*12 doubles (may be NULL). */
+int mbt_ring_prepare(mbt_ring *r, double *init_stats);
+/* Tracks nturns turns.  stats: nturns*Nbunch*12 doubles, ampinv: nturns*Nbunch*6
+ * (written on monitor turns only), rf: nturns*2 (phai0, Vrf0 used by the turn's map).
+ * Any output pointer may be NULL. */
+int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *ampinv, double *rf);
+/* Statistics after the last pending RW kick (src/worker_weakweak.c:397-405). */
+int mbt_ring_final_stats(mbt_ring *r, double *stats);
+/* Selffield histogram of the last completed turn (clamped rule,
+ * src/transform_weak.c:608-631) as counts[Nbunch*Ncell] plus trash counters [2*Nbunch]. */
+int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash);
+/* Kernel timing of the last mbt_ring_track call, milliseconds per kernel class:
+ * [0] turn map pass, [1] per-bunch reduction, [2] LR recurrence, [3] RW sum. */
+int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches);
+/* Bytes of HBM the turn map pass must move per turn (algorithmic). */
+double mbt_ring_bytes_per_turn(const mbt_ring *r);
+long mbt_ring_turn(const mbt_ring *r);
+
+/* ---- end-to-end drop-in ----------------------------------------------------- */
+/* Runs manager_weakweak + all workers of the reference on one GPU and writes the
+ * reference's output files into cfg->work_path. */
+int mbt_manager_weakweak(const mbt_config *cfg, int device);
+
+/* ---- Faddeeva function on the GPU ------------------------------------------- */
+/* w(z) for n points (x[i] + i y[i]); FP64. Returns status per point in flag (0 ok,
+ * 1 overflow, as complex_errorfunc_wofz_algo680 src/physic.c:75-301). */
+int mbt_wofz(int device, long n, const double *x, const double *y, double *wr, double *wi, int *flag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
