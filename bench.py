@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark: macroparticle-turns/s of the multi-bunch tracking turn on one MI355X.
+
+Workload (production ring): 3 GeV, 354 m, h = 416, 416 bunches x 100,000
+macroparticles, 100 longitudinal slices per bunch with integer counts,
+broadband resonator (10 GHz, Q 1, 5 kOhm) short-range wake, resistive-wall
+long-range wake over 416 buckets x 50 past turns, 0.5 mA/bunch, quantum
+excitation off, FP64.  A step is one turn of the whole ring with every
+coordinate resident in HBM; the initial distribution (reference generator) and
+set-up are outside the timed region.
+
+Prints one JSON line (see the fields below).  `roofline` describes the
+per-particle pass kernel k_turn: 96 algorithmic bytes per macroparticle-turn
+(read + write of six FP64 coordinates) over its average duration measured with
+HIP events on the ring's stream during the timed turns.  `cpu_baseline` times the
+reference's own code (oracle/_ref/refharness, built from the reference sources)
+or, when that binary is absent, the project's C restatement, on the same ring for
+a bounded number of turns.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_MP_TURN = 96.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_turn_production_traffic.txt")
+
+
+def production_conf(d, np_, turns):
+    from mtrack_amd import confgen
+    path = os.path.join(d, "production.conf")
+    confgen.write(path, confgen.wl_production(turns=turns, np_=np_))
+    return path
+
+
+def read_traffic():
+    """HBM bytes per k_turn launch measured with rocprofv3 --pmc (scaled by the
+    copy-kernel calibration), from the committed profile summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 2 and not line.startswith("#"):
+                try:
+                    vals[p[0]] = float(p[1])
+                except ValueError:
+                    pass
+    return vals.get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(np_, turns=2, timeout=240):
+    """Reference turn loop on the host (one core), same ring, `turns` turns."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "refharness")
+    with tempfile.TemporaryDirectory() as d:
+        conf = production_conf(d, np_, turns)
+        sample = "production ring, 416 bunches x %d mp x %d turns (turn loop only)" % (np_, turns)
+        if os.path.exists(harness):
+            r = subprocess.run([harness, conf, d, str(turns)], capture_output=True, text=True, timeout=timeout)
+            for line in r.stderr.splitlines():
+                if line.startswith("turn_loop_seconds"):
+                    sec = float(line.split()[1])
+                    return {"value": 416 * np_ * turns / sec, "unit": "macroparticle-turns/s", "cores": 1,
+                            "kind": "reference", "sample": sample}
+        import mtrack_amd as m
+        import oracle
+        cfg = m.Config(conf, d + "/")
+        orc = oracle.OracleRing(cfg)
+        orc.prepare()
+        t0 = time.perf_counter()
+        orc.track(turns)
+        sec = time.perf_counter() - t0
+        return {"value": 416 * np_ * turns / sec, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
+                "sample": sample}
+
+
+def copy_calibration():
+    """Plain device copy of a known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = 1 << 27  # 128 Mi doubles = 1 GiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--np", type=int, default=100000, dest="np_")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-turns", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true")
+    a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU (bunch-sharded) path is not part of it. "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+    import mtrack_amd as m
+    with tempfile.TemporaryDirectory() as d:
+        conf = production_conf(d, a.np_, 20000)
+        cfg = m.Config(conf, d + "/")
+        ring = m.Ring(cfg)
+        ring.prepare()
+        nb, npb = ring.Nb, ring.Np
+        if a.warmup > 0:
+            ring.track(a.warmup)
+        ring.set_timing(True)
+        t0 = time.perf_counter()
+        ring.track(a.steps)
+        t1 = time.perf_counter()
+        ms, launches = ring.kernel_times()
+        bytes_turn = ring.bytes_per_turn()
+        ring.close()
+    sec = t1 - t0
+    value = nb * npb * a.steps / sec
+    k_turn_ms = ms[0] / max(launches[0], 1)
+    achieved = bytes_turn / (k_turn_ms * 1e-3) / 1e9
+    out = {
+        "metric": "macroparticle-turns/s on 1 MI355X (FP64 vs mixed/batched) + % HBM peak",
+        "value": value,
+        "unit": "macroparticle-turns/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * sec / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: reference Gaussian generator (glibc rand stream), seeded",
+        "config": {"workload": "production ring FP64: 416 bunches x %d macroparticles, 100 slices, "
+                               "broadband resonator + resistive wall 416 x 50 turns, QE off" % npb,
+                   "bunches": nb, "macroparticles_per_bunch": npb, "turns_per_step": 1},
+        "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
+                               "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+    }
+    if not a.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(a.np_, a.cpu_turns)
+        except Exception as e:  # the GPU number stands on its own
+            out["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

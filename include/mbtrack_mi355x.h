@@ -161,6 +161,14 @@ const char *mbt_last_error(void);
 /* ---- host configuration (C) ------------------------------------------------ */
 int mbt_config_read(const char *conf_path, const char *work_path, mbt_config *cfg);
 void mbt_config_free(mbt_config *cfg);
+/* Heap-allocated configuration for bindings; mbt_config_release frees both. */
+mbt_config *mbt_config_alloc(void);
+void mbt_config_release(mbt_config *cfg);
+/* Scalar fields by name (e.g. "Nbunch", "T0", "TrackPlane_HOR"). */
+int mbt_config_get(const mbt_config *cfg, const char *name, double *value);
+int mbt_config_set(mbt_config *cfg, const char *name, double value);
+/* Bucket of the k-th filled bunch, -1 if out of range. */
+int mbt_config_bucket(const mbt_config *cfg, int k);
 /* Writes every scalar of the configuration as "name value" (%.17g), one per line. */
 int mbt_config_dump(const mbt_config *cfg, const char *path);
 /* Green functions Gl/Gv/Gh[Ncell] for the configuration (src/transform_weak.c:289-471);
@@ -168,6 +176,8 @@ int mbt_config_dump(const mbt_config *cfg, const char *path);
 int mbt_greens_functions(const mbt_config *cfg, double *Gl, double *Gv, double *Gh, int planes[3]);
 /* Initial coordinates of bucket kb, SoA: out[0..Np) tau, [Np..2Np) delta, x, xp, z, zp. */
 int mbt_generate_bunch(const mbt_config *cfg, int kb, double *out);
+/* All filled bunches into out[Nbunch][6][Np] (host threads over bunches). */
+int mbt_generate_all(const mbt_config *cfg, double *out);
 
 /* ---- the ring on the GPU ---------------------------------------------------- */
 typedef struct mbt_ring mbt_ring;
@@ -198,6 +208,8 @@ int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash);
 int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches);
 /* Bytes of HBM the turn map pass must move per turn (algorithmic). */
 double mbt_ring_bytes_per_turn(const mbt_ring *r);
+/* Per-kernel timing with HIP events around every launch group (0 = off). */
+int mbt_ring_set_timing(mbt_ring *r, int on);
 long mbt_ring_turn(const mbt_ring *r);
 
 /* ---- end-to-end drop-in ----------------------------------------------------- */

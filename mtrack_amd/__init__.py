@@ -1,0 +1,185 @@
+"""mtrack_amd: MI355X multi-bunch tracking turn behind a C ABI.
+
+The product is the shared library libmbtrack_mi355x.so (host C + gfx950 HIP
+kernels) declared in include/mbtrack_mi355x.h.  This module only binds it with
+ctypes for tests and benchmarks; it has no compute path of its own, and
+loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmbtrack_mi355x.so")
+EXE = os.path.join(HERE, "mbtrack-mi355x")
+
+NSTAT = 12
+NAMP = 6
+MBT_RING_NO_INIT_DIST = 1
+
+_lib = None
+
+
+class MbtError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise MbtError("libmbtrack_mi355x.so is not built: run python -m mtrack_amd.build")
+        L = ctypes.CDLL(LIBPATH)
+        vp, cp, ip, dp = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.mbt_last_error.restype = cp
+        L.mbt_config_alloc.restype = vp
+        L.mbt_config_release.argtypes = [vp]
+        L.mbt_config_read.argtypes = [cp, cp, vp]
+        L.mbt_config_get.argtypes = [vp, cp, dp]
+        L.mbt_config_set.argtypes = [vp, cp, ctypes.c_double]
+        L.mbt_config_bucket.argtypes = [vp, ip]
+        L.mbt_config_dump.argtypes = [vp, cp]
+        L.mbt_generate_bunch.argtypes = [vp, ip, dp]
+        L.mbt_greens_functions.argtypes = [vp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.mbt_ring_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
+        L.mbt_ring_destroy.argtypes = [vp]
+        L.mbt_ring_set_bunch.argtypes = [vp, ip, dp]
+        L.mbt_ring_get_bunch.argtypes = [vp, ip, dp]
+        L.mbt_ring_prepare.argtypes = [vp, dp]
+        L.mbt_ring_track.argtypes = [vp, ctypes.c_long, dp, dp, dp]
+        L.mbt_ring_final_stats.argtypes = [vp, dp]
+        L.mbt_ring_histograms.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.mbt_ring_kernel_times.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
+        L.mbt_ring_bytes_per_turn.argtypes = [vp]
+        L.mbt_ring_bytes_per_turn.restype = ctypes.c_double
+        L.mbt_ring_set_timing.argtypes = [vp, ip]
+        L.mbt_ring_turn.argtypes = [vp]
+        L.mbt_ring_turn.restype = ctypes.c_long
+        if hasattr(L, "mbt_manager_weakweak"):
+            L.mbt_manager_weakweak.argtypes = [vp, ip]
+        if hasattr(L, "mbt_wofz"):
+            L.mbt_wofz.argtypes = [ip, ctypes.c_long, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MbtError("mbtrack error %d: %s" % (rc, lib().mbt_last_error().decode()))
+
+
+def dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+class Config:
+    """A parsed .conf file with its derived machine parameters."""
+
+    def __init__(self, conf_path, work_path="work/"):
+        L = lib()
+        self.ptr = ctypes.c_void_p(L.mbt_config_alloc())
+        self.path = conf_path
+        _check(L.mbt_config_read(conf_path.encode(), work_path.encode(), self.ptr))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mbt_config_release(self.ptr)
+            self.ptr = None
+
+    def get(self, name):
+        v = ctypes.c_double()
+        _check(lib().mbt_config_get(self.ptr, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set(self, name, value):
+        _check(lib().mbt_config_set(self.ptr, name.encode(), float(value)))
+
+    def geti(self, name):
+        return int(self.get(name))
+
+    @property
+    def buckets(self):
+        return [lib().mbt_config_bucket(self.ptr, k) for k in range(self.geti("Nbunch"))]
+
+    def generate_bunch(self, kb):
+        n = self.geti("Np")
+        out = np.zeros(6 * n)
+        _check(lib().mbt_generate_bunch(self.ptr, kb, dptr(out)))
+        return out.reshape(6, n)
+
+    def greens(self):
+        nc = max(self.geti("Ncell"), 1)
+        g = [np.zeros(nc) for _ in range(3)]
+        planes = (ctypes.c_int * 3)()
+        _check(lib().mbt_greens_functions(self.ptr, dptr(g[0]), dptr(g[1]), dptr(g[2]), planes))
+        return g[0], g[1], g[2], list(planes)
+
+    def dump(self, path):
+        _check(lib().mbt_config_dump(self.ptr, path.encode()))
+
+
+class Ring:
+    """All bunches of one ring resident on one GPU (mbt_ring_* C ABI)."""
+
+    def __init__(self, cfg, device=0, init_dist=True):
+        L = lib()
+        self.cfg = cfg
+        self.Nb = cfg.geti("Nbunch")
+        self.Np = cfg.geti("Np")
+        self.Nc = cfg.geti("Ncell")
+        p = ctypes.c_void_p()
+        _check(L.mbt_ring_create(cfg.ptr, device, 0 if init_dist else MBT_RING_NO_INIT_DIST, ctypes.byref(p)))
+        self.ptr = p
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().mbt_ring_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def set_bunch(self, ib, soa6):
+        a = np.ascontiguousarray(soa6, dtype=np.float64).reshape(-1)
+        _check(lib().mbt_ring_set_bunch(self.ptr, ib, dptr(a)))
+
+    def get_bunch(self, ib):
+        a = np.zeros(6 * self.Np)
+        _check(lib().mbt_ring_get_bunch(self.ptr, ib, dptr(a)))
+        return a.reshape(6, self.Np)
+
+    def prepare(self):
+        st = np.zeros((self.Nb, NSTAT))
+        _check(lib().mbt_ring_prepare(self.ptr, dptr(st)))
+        return st
+
+    def track(self, nturns, want_stats=True, want_amp=True, want_rf=True):
+        st = np.zeros((nturns, self.Nb, NSTAT)) if want_stats else None
+        am = np.zeros((nturns, self.Nb, NAMP)) if want_amp else None
+        rf = np.zeros((nturns, 2)) if want_rf else None
+        _check(lib().mbt_ring_track(self.ptr, nturns, dptr(st), dptr(am), dptr(rf)))
+        return st, am, rf
+
+    def final_stats(self):
+        st = np.zeros((self.Nb, NSTAT))
+        _check(lib().mbt_ring_final_stats(self.ptr, dptr(st)))
+        return st
+
+    def histograms(self):
+        c = np.zeros((self.Nb, max(self.Nc, 1)))
+        t = (ctypes.c_int * (2 * self.Nb))()
+        _check(lib().mbt_ring_histograms(self.ptr, dptr(c), t))
+        return c, np.array(list(t)).reshape(self.Nb, 2)
+
+    def set_timing(self, on=True):
+        _check(lib().mbt_ring_set_timing(self.ptr, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        n = (ctypes.c_long * 4)()
+        _check(lib().mbt_ring_kernel_times(self.ptr, dptr(ms), n))
+        return ms, list(n)
+
+    def bytes_per_turn(self):
+        return lib().mbt_ring_bytes_per_turn(self.ptr)

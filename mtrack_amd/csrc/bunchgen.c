@@ -299,3 +299,54 @@ int mbt_greens_functions(const mbt_config *c, double *Gl, double *Gv, double *Gh
   }
   return MBT_OK;
 }
+
+/* ---------------------------------------- all bunches, one thread per bunch */
+
+#include <pthread.h>
+#include <unistd.h>
+
+typedef struct gen_job {
+  const mbt_config *c;
+  double *out;
+  int next;
+  int rc;
+  pthread_mutex_t mu;
+} gen_job;
+
+static void *gen_worker(void *arg)
+{
+  gen_job *j = (gen_job *)arg;
+  const size_t per = 6 * (size_t)j->c->Np;
+  for (;;) {
+    pthread_mutex_lock(&j->mu);
+    int b = j->next++;
+    int stop = j->rc != MBT_OK;
+    pthread_mutex_unlock(&j->mu);
+    if (stop || b >= j->c->Nbunch) break;
+    int rc = mbt_generate_bunch(j->c, j->c->bucket[b], j->out + per * b);
+    if (rc) {
+      pthread_mutex_lock(&j->mu);
+      j->rc = rc;
+      pthread_mutex_unlock(&j->mu);
+    }
+  }
+  return NULL;
+}
+
+/* Initial coordinates of every filled bunch into out[Nbunch][6][Np]; bunches are
+ * independent streams (one reference rank each), so they are generated by up to
+ * 16 host threads. */
+int mbt_generate_all(const mbt_config *c, double *out)
+{
+  long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+  int nt = ncpu > 16 ? 16 : (ncpu < 1 ? 1 : (int)ncpu);
+  if (nt > c->Nbunch) nt = c->Nbunch;
+  gen_job j = {c, out, 0, MBT_OK, PTHREAD_MUTEX_INITIALIZER};
+  pthread_t th[16];
+  int started = 0;
+  for (int i = 0; i < nt; i++)
+    if (pthread_create(&th[i], NULL, gen_worker, &j) == 0) started++;
+  if (started == 0) gen_worker(&j);
+  for (int i = 0; i < started; i++) pthread_join(th[i], NULL);
+  return j.rc;
+}

@@ -913,3 +913,86 @@ int mbt_config_dump(const mbt_config *c, const char *path)
   fclose(fp);
   return MBT_OK;
 }
+
+/* ------------------------------------------------- accessors for bindings */
+
+#include <stddef.h>
+
+typedef struct field_desc {
+  const char *name;
+  size_t off;
+  char kind; /* d double, i int, l long, u unsigned */
+} field_desc;
+
+#define FD(n, k) {#n, offsetof(mbt_config, n), k}
+static const field_desc g_fields[] = {
+    FD(Nbunch, 'i'), FD(Np, 'i'), FD(Ncell, 'i'), FD(Nharm, 'u'), FD(NrevTot, 'l'), FD(NrevMon, 'l'),
+    FD(NrevScan, 'l'), FD(Nscan, 'l'), FD(scan, 'i'), FD(Nmlt, 'i'), FD(EnableQuantum, 'i'),
+    FD(EnableRW_long, 'i'), FD(EnableIdealHC, 'i'), FD(EnableAmpinv_out, 'i'), FD(bunch_model, 'i'),
+    FD(Nbunch_out, 'i'), FD(has_rf_feedback, 'i'), FD(n_res, 'i'), FD(NrevPotentialsOut, 'l'),
+    FD(T0, 'd'), FD(h, 'd'), FD(E0, 'd'), FD(sigma_tau, 'd'), FD(Nsigma, 'd'), FD(dT, 'd'), FD(taue, 'd'),
+    FD(taux, 'd'), FD(tauz, 'd'), FD(De, 'd'), FD(Urad, 'd'), FD(phai0, 'd'), FD(Vrf0, 'd'), FD(fNp, 'd'),
+    FD(QH0, 'd'), FD(QV0, 'd'), FD(Qso, 'd'), FD(ac, 'd'), FD(Iring, 'd'), FD(Lc, 'd'), FD(rhorw, 'd'),
+    FD(scan_start, 'd'), FD(scan_step, 'd'), FD(Gzix, 'd'), FD(Gziz, 'd'), FD(wrf, 'd'), FD(Nslc, 'i'),
+};
+#undef FD
+
+static const field_desc *find_field(const char *name)
+{
+  for (size_t i = 0; i < sizeof g_fields / sizeof g_fields[0]; i++)
+    if (strcmp(g_fields[i].name, name) == 0) return &g_fields[i];
+  if (strcmp(name, "TrackPlane_HOR") == 0) {
+    static field_desc f = {"TrackPlane_HOR", offsetof(mbt_config, TrackPlane) + sizeof(int), 'i'};
+    return &f;
+  }
+  if (strcmp(name, "TrackPlane_VER") == 0) {
+    static field_desc f = {"TrackPlane_VER", offsetof(mbt_config, TrackPlane) + 2 * sizeof(int), 'i'};
+    return &f;
+  }
+  return NULL;
+}
+
+mbt_config *mbt_config_alloc(void) { return (mbt_config *)calloc(1, sizeof(mbt_config)); }
+
+void mbt_config_release(mbt_config *c)
+{
+  if (!c) return;
+  mbt_config_free(c);
+  free(c);
+}
+
+int mbt_config_get(const mbt_config *c, const char *name, double *v)
+{
+  const field_desc *f = find_field(name);
+  if (!f) {
+    mbt_set_error("unknown configuration field %s", name);
+    return MBT_EINVAL;
+  }
+  const char *p = (const char *)c + f->off;
+  switch (f->kind) {
+  case 'd': *v = *(const double *)p; break;
+  case 'i': *v = *(const int *)p; break;
+  case 'u': *v = *(const unsigned *)p; break;
+  default: *v = (double)*(const long *)p; break;
+  }
+  return MBT_OK;
+}
+
+int mbt_config_set(mbt_config *c, const char *name, double v)
+{
+  const field_desc *f = find_field(name);
+  if (!f) {
+    mbt_set_error("unknown configuration field %s", name);
+    return MBT_EINVAL;
+  }
+  char *p = (char *)c + f->off;
+  switch (f->kind) {
+  case 'd': *(double *)p = v; break;
+  case 'i': *(int *)p = (int)v; break;
+  case 'u': *(unsigned *)p = (unsigned)v; break;
+  default: *(long *)p = (long)v; break;
+  }
+  return MBT_OK;
+}
+
+int mbt_config_bucket(const mbt_config *c, int k) { return (k >= 0 && k < c->Nbunch) ? c->bucket[k] : -1; }

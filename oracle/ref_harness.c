@@ -37,6 +37,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <time.h>
 #include "types.h"
 #include "bunch.h"
 #include "input.h"
@@ -232,6 +233,8 @@ int main(int argc, char **argv)
 
   int lr_tot = nlrL + nlrH + nlrV;
   int *Nstat = calloc(Nb, sizeof(int));
+  struct timespec t_a, t_b;
+  clock_gettime(CLOCK_MONOTONIC, &t_a);
   for (long rev = 0; rev < track.NrevTot; rev++) {
     for (int k = 0; k < Nb; k++) {
       worker_t *w = &W[k];
@@ -308,6 +311,8 @@ int main(int argc, char **argv)
         W[k].m = 0;
       }
   }
+  clock_gettime(CLOCK_MONOTONIC, &t_b);
+  fprintf(stderr, "turn_loop_seconds %.6f\n", (t_b.tv_sec - t_a.tv_sec) + 1e-9 * (t_b.tv_nsec - t_a.tv_nsec));
   if (track.scan == 0)
     for (int k = 0; k < Nb; k++) {
       weak_bunch_calc_statistics(&W[k].bunch, &track);

@@ -1,0 +1,100 @@
+"""GPU turn vs the CPU oracle on the same seeded inputs, through the C ABI.
+
+Tolerances (FP64 everywhere on both sides):
+  * slice counts and trash counters: bit-exact;
+  * bunch rms and ampinv: relative error <= RTOL;
+  * centroids: |gpu - oracle| <= RTOL * rms of the same coordinate (centroids of
+    a centred bunch are ~1e-3 sigma, so a relative error of the centroid itself is
+    not meaningful);
+  * RF phase / voltage: relative error <= RTOL.
+The residual difference comes from the device's sin/cos (ocml, <= 1 ulp) against
+glibc's, and from the summation order of moments and of the RW sum.  The QE
+normals are bit-identical on both sides (counter-based stream, explicit-FMA
+float Box-Muller), so QE-on runs are held to the same bar.
+"""
+import numpy as np
+import pytest
+
+import mtrack_amd as m
+import oracle as o
+from variants import VARIANTS, write_variant
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-11
+
+
+def stat_errors(g, r):
+    """Per-column normalised error of stats arrays [..., 12]."""
+    g = np.asarray(g)
+    r = np.asarray(r)
+    err = np.zeros(g.shape)
+    for p in range(3):
+        for mean_c, sig_c in ((p, 3 + p), (6 + p, 9 + p)):
+            scale = np.maximum(np.abs(r[..., sig_c]), 1e-300)
+            err[..., mean_c] = np.abs(g[..., mean_c] - r[..., mean_c]) / scale
+            err[..., sig_c] = np.abs(g[..., sig_c] - r[..., sig_c]) / scale
+    return err
+
+
+def run_pair(path, work, turns=None):
+    cfg = m.Config(path, work)
+    n = turns or cfg.geti("NrevTot")
+    orc = o.OracleRing(cfg, o.RNG_PHILOX)
+    ost0 = orc.prepare()
+    ost, oam, orf = orc.track(n)
+    g = m.Ring(cfg)
+    gst0 = g.prepare()
+    gst, gam, grf = g.track(n)
+    return cfg, (orc, ost0, ost, oam, orf), (g, gst0, gst, gam, grf)
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_turns_match_oracle(tmp_path, name):
+    path = write_variant(name, str(tmp_path))
+    print("case", name, flush=True)
+    cfg, (orc, ost0, ost, oam, orf), (g, gst0, gst, gam, grf) = run_pair(path, str(tmp_path) + "/")
+    e0 = stat_errors(gst0, ost0).max()
+    e = stat_errors(gst, ost)
+    print("%s: init err %.3g, turn err max %.3g (per column %s)" % (
+        name, e0, e.max(), np.array2string(e.reshape(-1, 12).max(0), precision=2)))
+    assert e0 <= RTOL
+    assert e.max() <= RTOL
+    mon = (np.arange(len(ost)) + 1) % cfg.geti("NrevMon") == 0
+    a_o, a_g = oam[mon], gam[mon]
+    ea = np.abs(a_g - a_o) / np.maximum(np.abs(a_o), 1e-300)
+    ea[np.abs(a_o) == 0] = np.abs(a_g[np.abs(a_o) == 0])
+    print("%s: ampinv err %.3g" % (name, ea.max()))
+    assert ea.max() <= RTOL
+    er = np.abs(grf - orf) / np.abs(orf)
+    assert er.max() <= RTOL
+    if cfg.geti("Ncell") > 0:
+        hc_o, tr_o = orc.histograms()
+        hc_g, tr_g = g.histograms()
+        assert np.array_equal(hc_o, hc_g), "slice counts differ"
+        assert np.array_equal(tr_o, tr_g), "trash counters differ"
+    fo = orc.final_stats()
+    fg = g.final_stats()
+    assert stat_errors(fg, fo).max() <= RTOL
+    # coordinates themselves
+    for ib in (0, g.Nb - 1):
+        co = orc.get_bunch(ib)
+        cg_ = g.get_bunch(ib)
+        sc = np.maximum(np.std(co, axis=1, keepdims=True), 1e-300)
+        assert (np.abs(cg_ - co) / sc).max() <= 1e-9
+    g.close()
+
+
+def test_deterministic_bitwise(tmp_path):
+    """Two GPU runs of the same input are bit-identical (exact integer histogram
+    sums, fixed-order moment reductions)."""
+    path = write_variant("wakes", str(tmp_path))
+    out = []
+    for _ in range(2):
+        cfg = m.Config(path, str(tmp_path) + "/")
+        g = m.Ring(cfg)
+        g.prepare()
+        st, am, rf = g.track(20)
+        out.append((st, am, g.get_bunch(3)))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

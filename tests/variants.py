@@ -1,0 +1,79 @@
+"""Small seeded ring configurations shared by the parity tests and the golden
+fixture generator.  Each exercises one part of the reference's weak-weak turn."""
+import os
+
+from mtrack_amd import confgen as cg
+
+L20 = cg.small_ring(20)
+RW = {"rhorw": 1.7e-8, "beffL1": 0.012, "beffL2": 0.012, "beffH3": 0.012, "beffV3": 0.012}
+BBR_L = ("resonator_1", {"Rs": 5000.0, "fres": 10.0, "Qfactor": 1.0, "plane": "LON"})
+
+
+def _ring(**kw):
+    d = {"Lc": L20, "Iring": 40.0}
+    d.update(RW)
+    d.update(kw)
+    return d
+
+
+VARIANTS = {
+    # broadband resonators in all planes + RW long-range, quantum excitation off
+    "wakes": dict(sections={"selffield": {"Ncell": 50, "Nsigma": 5.0},
+                            "tracking": {"NrevTot": 60, "NrevMon": 10, "EnableRW_long": 1, "Nmlt": 3},
+                            "ring": _ring(), "distribution": {"Np": 1000}},
+                  extra_sections=[BBR_L,
+                                  ("resonator_2", {"Rs": 2e5, "fres": 8.0, "Qfactor": 1.0, "plane": "VER"}),
+                                  ("resonator_3", {"Rs": 2e5, "fres": 8.0, "Qfactor": 1.0, "plane": "HOR"})]),
+    # quantum excitation and radiation damping
+    "qe": dict(sections={"tracking": {"NrevTot": 40, "EnableQuantum": 1}, "ring": _ring(),
+                         "distribution": {"Np": 500}}),
+    # harmonic cavities (main + 3rd) as long-range resonators with RF feedback
+    "lr_fb": dict(sections={"selffield": {"Ncell": 40, "Nsigma": 5.0}, "tracking": {"NrevTot": 30},
+                            "ring": _ring(Iring=200.0), "distribution": {"Np": 400}},
+                  extra_sections=[("harmonic_cavity_1", {"m": 1, "Rs": 5e6, "detune": -40e3, "Qfactor": 4e4}),
+                                  ("harmonic_cavity_2", {"m": 3, "Rs": 2e6, "detune": 50e3, "Qfactor": 2e4}),
+                                  ("rf_feedback", {"resonator": 1, "averaging_length": 30})]),
+    # transverse long-range resonators (HOMs) + LON broadband, QE on
+    "lr_hv": dict(sections={"selffield": {"Ncell": 40, "Nsigma": 5.0},
+                            "tracking": {"NrevTot": 30, "EnableQuantum": 1},
+                            "ring": _ring(), "distribution": {"Np": 400}},
+                  extra_sections=[("longrange_resonator_1", {"plane": "HOR", "fres": 1.2, "Rs": 1e6, "Qfactor": 3e3}),
+                                  ("longrange_resonator_2", {"plane": "VER", "fres": 1.3, "Rs": 1e6, "Qfactor": 3e3}),
+                                  BBR_L]),
+    # active harmonic cavity, amplitude-dependent tunes, optics with alpha and dispersion
+    "ahc_amp": dict(sections={"tracking": {"NrevTot": 30, "EnableQuantum": 1},
+                              "ring": _ring(CHH=100.0, CHV=50.0, CVH=30.0, CVV=70.0, alphaH=0.5, alphaV=-0.3,
+                                            dispH=0.1, disppH=0.01),
+                              "distribution": {"Np": 300}},
+                    extra_sections=[("active_HC_1", {"Vpeak": 3e5, "nHC": 3.0, "phi_aHC": 0.3})]),
+    # ideal harmonic cavity
+    "ideal": dict(sections={"tracking": {"NrevTot": 30, "EnableIdealHC": 1, "EnableQuantum": 1},
+                            "ring": _ring(mIdeal=3), "distribution": {"Np": 300}}),
+    # 64 bunches in 416 buckets from a filling file, RW trigger, automatic Nmlt
+    "fromfile": dict(sections={"selffield": {"Ncell": 60, "Nsigma": 6.0},
+                               "tracking": {"NrevTot": 40, "filling": "@FILL@", "EnableRW_long": 1, "Nmlt": 0,
+                                            "triggerRW": 1},
+                               "ring": dict(Iring=30.0, **RW), "distribution": {"Np": 200}},
+                     extra_sections=[BBR_L]),
+    # current scan (scan_type 1)
+    "scan": dict(sections={"selffield": {"Ncell": 40, "Nsigma": 5.0},
+                           "tracking": {"NrevTot": 30, "EnableRW_long": 1, "Nmlt": 2},
+                           "ring": _ring(), "distribution": {"Np": 200}},
+                 extra_sections=[BBR_L, ("scan", {"scan_type": 1, "Nscan": 3, "NrevScan": 10, "scan_start": 20.0,
+                                                  "scan_step": 10.0})]),
+    # resistive + inductive smoothing terms (bunch well inside the binning range)
+    "rsis": dict(sections={"selffield": {"Ncell": 60, "Nsigma": 9.0, "RsisL": 300.0, "aindL": 5.0},
+                           "tracking": {"NrevTot": 30}, "ring": _ring(), "distribution": {"Np": 400}},
+                 extra_sections=[BBR_L]),
+}
+
+FILL_BUCKETS = [int(k * 6.5) for k in range(64)]
+
+
+def write_variant(name, d):
+    """Writes the variant's .conf (and filling file) into directory d; returns the path."""
+    os.makedirs(d, exist_ok=True)
+    fill = os.path.join(d, "fill.txt")
+    cg.filling_file(fill, FILL_BUCKETS, out=(0, 13))
+    text = cg.ring_conf(**VARIANTS[name]).replace("@FILL@", os.path.abspath(fill))
+    return cg.write(os.path.join(d, name + ".conf"), text)
