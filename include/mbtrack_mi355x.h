@@ -169,6 +169,12 @@ int mbt_config_get(const mbt_config *cfg, const char *name, double *value);
 int mbt_config_set(mbt_config *cfg, const char *name, double value);
 /* Bucket of the k-th filled bunch, -1 if out of range. */
 int mbt_config_bucket(const mbt_config *cfg, int k);
+/* One-key lookups with the reader's grammar; 1 found, 0 absent or unparsable, <0 error.
+ * Replace config_get_double/_int/_str and config_have_section, src/input.c:972-1100. */
+int mbt_conf_get_double(const char *path, const char *section, const char *key, double *value);
+int mbt_conf_get_int(const char *path, const char *section, const char *key, int *value);
+int mbt_conf_get_str(const char *path, const char *section, const char *key, char *value, int len);
+int mbt_conf_have_section(const char *path, const char *section);
 /* Writes every scalar of the configuration as "name value" (%.17g), one per line. */
 int mbt_config_dump(const mbt_config *cfg, const char *path);
 /* Green functions Gl/Gv/Gh[Ncell] for the configuration (src/transform_weak.c:289-471);

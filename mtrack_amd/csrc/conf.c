@@ -996,3 +996,43 @@ int mbt_config_set(mbt_config *c, const char *name, double v)
 }
 
 int mbt_config_bucket(const mbt_config *c, int k) { return (k >= 0 && k < c->Nbunch) ? c->bucket[k] : -1; }
+
+/* ------------------------------------------ single-key lookups (src/input.c:972-1100) */
+
+static int lookup(const char *path, const char *sec, const char *key, int kind, void *out, size_t outlen)
+{
+  conf_text T;
+  if (conf_load(path, &T) != 0) {
+    mbt_set_error("cannot open %s", path);
+    return MBT_EIO;
+  }
+  T.print_errors = 0;
+  int ok = conf_get(&T, sec, key, kind, out, outlen);
+  conf_free(&T);
+  return ok ? 1 : 0;
+}
+
+int mbt_conf_get_double(const char *path, const char *section, const char *key, double *v)
+{
+  return lookup(path, section, key, K_DOUBLE, v, 0);
+}
+
+int mbt_conf_get_int(const char *path, const char *section, const char *key, int *v)
+{
+  return lookup(path, section, key, K_INT, v, 0);
+}
+
+int mbt_conf_get_str(const char *path, const char *section, const char *key, char *v, int len)
+{
+  return lookup(path, section, key, K_STR, v, (size_t)len);
+}
+
+int mbt_conf_have_section(const char *path, const char *section)
+{
+  conf_text T;
+  if (conf_load(path, &T) != 0) return MBT_EIO;
+  T.print_errors = 0;
+  int ok = have_section(&T, section);
+  conf_free(&T);
+  return ok;
+}

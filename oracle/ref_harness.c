@@ -113,6 +113,49 @@ int main(int argc, char **argv)
   if (nturns_override > 0) track.NrevTot = nturns_override;
   if (track.scan == 0) track.NrevScan = track.NrevTot + 1;
 
+  {
+    /* derived parameters, same names as the project's mbt_config_dump */
+    char pp[FILENAME_MAX + 64];
+    snprintf(pp, sizeof pp, "%s/params.txt", track.work_path);
+    FILE *fpp = fopen(pp, "w");
+#define PD(name, v) fprintf(fpp, "%s %.17g\n", name, (double)(v))
+    PD("Nharm", ring.Nharm); PD("h", ring.h); PD("T0", ring.T0); PD("w0", ring.w0); PD("wrf", ring.wrf);
+    PD("R", ring.R); PD("Gamma", ring.Gamma); PD("Gamma2", ring.Gamma2); PD("Gamma3", ring.Gamma3);
+    PD("U0", ring.U0); PD("Urad", ring.Urad); PD("De", ring.De); PD("taue", ring.taue); PD("tauC", ring.tauC);
+    PD("aexpe", ring.aexpe); PD("Je", ring.Je); PD("rho0", ring.rho0); PD("q", ring.q); PD("Fq", ring.Fq);
+    PD("epsMax", ring.epsMax); PD("Vrfp", ring.Vrfp); PD("wso", ring.wso); PD("wso2", ring.wso2);
+    PD("fso", ring.fso); PD("Qso", ring.Qso); PD("wgziH", ring.wgziH); PD("wgziV", ring.wgziV);
+    PD("sn0", ring.sn0); PD("phai0", ring.phai0); PD("Vrf0", ring.Vrf0); PD("fc1", ring.fc1); PD("fc12", ring.fc12);
+    PD("Jx", ring.Jx); PD("Jz", ring.Jz); PD("taux", ring.taux); PD("tauz", ring.tauz);
+    PD("gamma1[1]", ring.gamma1[1]); PD("gamma1[2]", ring.gamma1[2]); PD("Gzix", ring.Gzix); PD("Gziz", ring.Gziz);
+    PD("CHH", ring.CHH); PD("CHV", ring.CHV); PD("CVH", ring.CVH); PD("CVV", ring.CVV);
+    PD("AmpDependdQH", ring.AmpDependdQH); PD("AmpDependdQV", ring.AmpDependdQV);
+    PD("m_aHC", ring.m_aHC); PD("phi_n", ring.phi_n); PD("HC_k", ring.HC_k); PD("Nbumax", ring.Nbumax);
+    PD("rhorw", ring.rhorw); PD("beffH[0]", ring.beffH[0]); PD("beffV[0]", ring.beffV[0]);
+    PD("NrevTot", track.NrevTot); PD("NrevMon", track.NrevMon); PD("NrevScan", track.NrevScan);
+    PD("Nscan", track.Nscan); PD("scan", track.scan); PD("scan_start", track.scan_start);
+    PD("scan_step", track.scan_step); PD("Nmlt", track.Nmlt); PD("NmultiT", track.NmultiT);
+    PD("triggerRW", track.triggerRW); PD("EnableRW_long", track.EnableRW_long);
+    PD("EnableQuantum", track.EnableQuantum); PD("EnableIdealHC", track.EnableIdealHC);
+    PD("Np", mp.Np); PD("iseed[0]", mp.iseed[0]); PD("iseed[1]", mp.iseed[1]); PD("iseed[2]", mp.iseed[2]);
+    for (int i = 0; i < 3; i++) {
+      char nm[32];
+      snprintf(nm, sizeof nm, "pos_offset[%d]", i); PD(nm, mp.pos_offset.v[i]);
+      snprintf(nm, sizeof nm, "slope_offset[%d]", i); PD(nm, mp.slope_offset.v[i]);
+      snprintf(nm, sizeof nm, "pos_sgm[%d]", i); PD(nm, mp.pos_sgm.v[i]);
+      snprintf(nm, sizeof nm, "slope_sgm[%d]", i); PD(nm, mp.slope_sgm.v[i]);
+      snprintf(nm, sizeof nm, "correlate[%d]", i); PD(nm, mp.correlate.v[i]);
+    }
+    PD("Ncell", sf0.Ncell); PD("Nsigma", sf0.Nsigma); PD("Nbunch", ebeam.Nbunch); PD("Nbunch_out", track.Nbunch_out);
+    PD("has_rf_feedback", ring.has_rf_feedback);
+    if (ring.has_rf_feedback) {
+      PD("rffb_vrf_design", ring.rf_feedback->vrf_design);
+      PD("rffb_phi0_design", ring.rf_feedback->phi0_design);
+    }
+    for (int i = 0; i < track.Nbunch_out; i++) fprintf(fpp, "bunch_out %d\n", track.bunch_out[i]);
+#undef PD
+    fclose(fpp);
+  }
   const int h = ring.Nharm;
   const int Nb = ebeam.Nbunch;
   /* manager: currents (manager_weakweak.c:58-93) */
@@ -125,6 +168,14 @@ int main(int argc, char **argv)
     ring.Ibunch[bnumbers[k]] = bunch_Ib[k] * FKILO;
   }
   mp.fNp = (double)mp.Np;
+  {
+    char pp[FILENAME_MAX + 64];
+    snprintf(pp, sizeof pp, "%s/params.txt", track.work_path);
+    FILE *fpp = fopen(pp, "a");
+    for (int i = 0; i < h; i++)
+      if (ebeam.nfFill[i]) fprintf(fpp, "Ibunch_%d %.17g\n", i, ring.Ibunch[i]);
+    fclose(fpp);
+  }
 
   char path[FILENAME_MAX + 64];
   snprintf(path, sizeof path, "%s/stats.txt", track.work_path);

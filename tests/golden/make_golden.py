@@ -1,0 +1,73 @@
+"""Writes the golden fixtures tests/golden/<variant>.npz from the reference's own
+code: oracle/_ref/refharness (reference translation units compiled from
+/root/reference/src by oracle/Makefile) run on every configuration of
+tests/variants.py.  Run in the build container only:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each fixture holds the .conf text it was made from, the per-turn per-bunch
+statistics (turn -1 = initial distribution, last row block = final statistics),
+RF phase/voltage per turn, ampinv on monitor turns, the slice histogram of
+selected turns, the final coordinates of the first bunch, and the derived
+machine parameters.  All values are the reference's doubles printed %.17g.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from variants import VARIANTS, write_variant  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "refharness")
+
+
+def load_params(path):
+    names, vals = [], []
+    with open(path) as f:
+        for line in f:
+            k, v = line.split()
+            names.append(k)
+            vals.append(float(v))
+    return np.array(names), np.array(vals)
+
+
+def make(name):
+    with tempfile.TemporaryDirectory() as d:
+        conf = write_variant(name, d)
+        text = open(conf).read().replace(os.path.abspath(os.path.join(d, "fill.txt")), "@FILL@")
+        first = None
+        r = subprocess.run([HARNESS, conf, d, "0", "1", "-1"], capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        st = np.loadtxt(os.path.join(d, "stats.txt"), ndmin=2)
+        first = int(st[0, 1])
+        r = subprocess.run([HARNESS, conf, d, "0", "1", str(first)], capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        st = np.loadtxt(os.path.join(d, "stats.txt"), ndmin=2)
+        rf = np.loadtxt(os.path.join(d, "rf.txt"), ndmin=2)
+        am = np.loadtxt(os.path.join(d, "ampinv.txt"), ndmin=2)
+        hp = os.path.join(d, "hist.txt")
+        hist = np.loadtxt(hp, ndmin=2) if os.path.exists(hp) and os.path.getsize(hp) else np.zeros((0, 4))
+        if hist.size:
+            keep = (hist[:, 0] % 10 == 9) | (hist[:, 0] == hist[:, 0].max())
+            hist = hist[keep]
+        parts = np.loadtxt(os.path.join(d, "parts_%d.txt" % first), ndmin=2)
+        pn, pv = load_params(os.path.join(d, "params.txt"))
+    out = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(out, conf=np.array(text), stats=st, rf=rf, ampinv=am, hist=hist, parts=parts,
+                        param_names=pn, param_values=pv)
+    return out
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or sorted(VARIANTS)):
+        p = make(n)
+        print(n, os.path.getsize(p))
