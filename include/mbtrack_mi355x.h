@@ -209,6 +209,14 @@ int mbt_ring_final_stats(mbt_ring *r, double *stats);
 /* Selffield histogram of the last completed turn (clamped rule,
  * src/transform_weak.c:608-631) as counts[Nbunch*Ncell] plus trash counters [2*Nbunch]. */
 int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash);
+/* Selffield tables the next turn will use for bunch ordinal ib, 7 rows of Ncell:
+ * factG*GL1, LR wake, slice counts, -factG*GlambdaV, dipoleV, -factG*GlambdaH,
+ * dipoleH (the columns of potentials_bunch_<kb>.dat, src/transform_weak.c:735-768);
+ * extra[3]: phase and voltage before the next turn's RF-feedback solve, scan value. */
+int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, double *extra);
+/* RW dipole history [(Nmlt+2)*h][2] in the reference's cyclic layout
+ * (src/worker_weakweak.c:203-219). */
+int mbt_ring_rw_history(const mbt_ring *r, double *out);
 /* Kernel timing of the last mbt_ring_track call, milliseconds per kernel class:
  * [0] turn map pass, [1] per-bunch reduction, [2] LR recurrence, [3] RW sum. */
 int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches);
@@ -227,6 +235,8 @@ int mbt_manager_weakweak(const mbt_config *cfg, int device);
 /* w(z) for n points (x[i] + i y[i]); FP64. Returns status per point in flag (0 ok,
  * 1 overflow, as complex_errorfunc_wofz_algo680 src/physic.c:75-301). */
 int mbt_wofz(int device, long n, const double *x, const double *y, double *wr, double *wi, int *flag);
+/* The same in FP32 (the mixed-precision beam-ion field); no reference counterpart. */
+int mbt_wofz_f32(int device, long n, const float *x, const float *y, float *wr, float *wi, int *flag);
 
 #ifdef __cplusplus
 }

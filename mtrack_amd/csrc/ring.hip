@@ -81,6 +81,7 @@ struct TurnState {
   int m;
   int has_fb;
   double phai0, Vrf0;
+  double phai0_prev, Vrf0_prev; // before this turn's RF-feedback solve
   double Iring, scan_val;
   long long fb_count;
 };
@@ -874,6 +875,8 @@ __global__ void k_fb_solve(TurnState *ts, const double *fb_hist, int fb_len, dou
                            double phi0_design, int switch_off)
 {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ts->phai0_prev = ts->phai0;
+  ts->Vrf0_prev = ts->Vrf0;
   if (!ts->has_fb) return;
   double vs = 0, ps = 0;
   for (int m = 0; m < fb_len; m++) {
@@ -1080,7 +1083,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
   for (int i = 0; i < 8; i++)
     if (r->ev[i]) hipEventDestroy(r->ev[i]);
   if (r->evpool) {
-    for (int i = 0; i < 5 * STATS_TURNS; i++) hipEventDestroy(r->evpool[i]);
+    for (int i = 0; i < 6 * STATS_TURNS; i++) hipEventDestroy(r->evpool[i]);
     free(r->evpool);
   }
   if (r->stream) hipStreamDestroy(r->stream);
@@ -1417,6 +1420,17 @@ static int launch_fb_solve(mbt_ring *r, int switch_off)
   return MBT_OK;
 }
 
+// Long-range recurrence and RF-feedback solve that precede a turn.
+static int pre_turn(mbt_ring *r)
+{
+  int rc;
+  if (r->P.sf_active && (rc = launch_lr(r, 0))) return rc;
+  hipLaunchKernelGGL(k_fb_solve, dim3(1), dim3(64), 0, r->stream, r->ts, r->fb_hist, r->fb_len, r->cfg.E0,
+                     r->cfg.rffb_vrf_design, r->cfg.rffb_phi0_design, 0);
+  CK(hipGetLastError());
+  return MBT_OK;
+}
+
 static int download_slots(mbt_ring *r, long long rev0, long n, double *stats, double *ampinv, double *rf)
 {
   CK(hipStreamSynchronize(r->stream));
@@ -1464,6 +1478,7 @@ extern "C" int mbt_ring_prepare(mbt_ring *r, double *init_stats)
     }
     if ((rc = launch_fb_solve(r, r->fb_off))) return rc;
   }
+  if ((rc = pre_turn(r))) return rc;
   CK(hipStreamSynchronize(r->stream));
   r->rev_host = 0;
   return MBT_OK;
@@ -1484,10 +1499,7 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
     long long rev0 = r->rev_host;
     const bool timed = r->timing != 0;
     for (long t = 0; t < n; t++) {
-      hipEvent_t *E = timed ? r->evpool + 5 * t : NULL;
-      if (timed) CK(hipEventRecord(E[0], r->stream));
-      if (r->P.sf_active && (rc = launch_lr(r, 0))) return rc;
-      if ((rc = launch_fb_solve(r, 0))) return rc;
+      hipEvent_t *E = timed ? r->evpool + 6 * t : NULL;
       hipLaunchKernelGGL(k_record_rf, dim3(1), dim3(64), 0, r->stream, r->ts, r->rfbuf);
       if (timed) CK(hipEventRecord(E[1], r->stream));
       if ((rc = launch_turn_pass(r, MODE_TURN))) return rc;
@@ -1503,17 +1515,23 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
       hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                          r->cfg.scan, (long long)r->cfg.NrevScan, r->cfg.scan_step);
       CK(hipGetLastError());
+      // the next turn's long-range wakes and RF-feedback solve (the reference runs
+      // them at the start of the turn; nothing changes the inputs in between)
+      if (timed) CK(hipEventRecord(E[0], r->stream));
+      if ((rc = pre_turn(r))) return rc;
+      if (timed) CK(hipEventRecord(E[5], r->stream));
       r->rev_host++;
     }
     if (timed) {
       CK(hipStreamSynchronize(r->stream));
       for (long t = 0; t < n; t++) {
-        hipEvent_t *E = r->evpool + 5 * t;
+        hipEvent_t *E = r->evpool + 6 * t;
         float a, b, c, d;
-        CK(hipEventElapsedTime(&a, E[0], E[1]));
+        CK(hipEventElapsedTime(&a, E[0], E[5]));
         CK(hipEventElapsedTime(&b, E[1], E[2]));
         CK(hipEventElapsedTime(&c, E[2], E[3]));
         CK(hipEventElapsedTime(&d, E[3], E[4]));
+        (void)0;
         r->kt_ms[2] += a;
         r->kt_ms[0] += b;
         r->kt_ms[1] += c;
@@ -1566,6 +1584,34 @@ extern "C" int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash
   return MBT_OK;
 }
 
+extern "C" int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, double *rfprev)
+{
+  if (ib < 0 || ib >= r->Nb || r->Nc <= 0) return MBT_EINVAL;
+  CK(hipSetDevice(r->device));
+  CK(hipStreamSynchronize(r->stream));
+  const size_t Nc = r->Nc, off = (size_t)ib * Nc, loff = (size_t)r->cfg.bucket[ib] * Nc;
+  const double *src[7] = {r->sfL + off, r->lr_wake + loff, r->fnp_out + off, r->sfV + off, r->dipV_out + off,
+                          r->sfH + off, r->dipH_out + off};
+  for (int k = 0; k < 7; k++) CK(hipMemcpy(out + k * Nc, src[k], Nc * sizeof(double), hipMemcpyDeviceToHost));
+  TurnState ts;
+  CK(hipMemcpy(&ts, r->ts, sizeof ts, hipMemcpyDeviceToHost));
+  if (rfprev) {
+    rfprev[0] = ts.phai0_prev;
+    rfprev[1] = ts.Vrf0_prev;
+    rfprev[2] = ts.scan_val;
+  }
+  return MBT_OK;
+}
+
+extern "C" int mbt_ring_rw_history(const mbt_ring *r, double *out)
+{
+  if (!r->cfg.EnableRW_long) return MBT_EINVAL;
+  CK(hipSetDevice(r->device));
+  CK(hipStreamSynchronize(r->stream));
+  CK(hipMemcpy(out, r->rwhist, (size_t)(r->cfg.Nmlt + 2) * r->h * 2 * sizeof(double), hipMemcpyDeviceToHost));
+  return MBT_OK;
+}
+
 extern "C" int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches)
 {
   for (int i = 0; i < 4; i++) {
@@ -1582,8 +1628,8 @@ extern "C" int mbt_ring_set_timing(mbt_ring *r, int on)
 {
   CK(hipSetDevice(r->device));
   if (on && !r->evpool) {
-    r->evpool = (hipEvent_t *)calloc(5 * STATS_TURNS, sizeof(hipEvent_t));
-    for (int i = 0; i < 5 * STATS_TURNS; i++) CK(hipEventCreate(&r->evpool[i]));
+    r->evpool = (hipEvent_t *)calloc(6 * STATS_TURNS, sizeof(hipEvent_t));
+    for (int i = 0; i < 6 * STATS_TURNS; i++) CK(hipEventCreate(&r->evpool[i]));
   }
   r->timing = on;
   return MBT_OK;

@@ -67,7 +67,53 @@ def make(name):
     return out
 
 
+
+
+MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def make_e2e(name="wakes"):
+    """Output files of the unmodified reference program (mbtrack-mpi under
+    MPICH, one rank per bunch + manager) for one variant, kept under
+    tests/golden/e2e_<name>/ for the drop-in driver's file-level comparison."""
+    import shutil
+    out = os.path.join(HERE, "e2e_" + name)
+    with tempfile.TemporaryDirectory() as d:
+        conf = write_variant(name, d)
+        work = os.path.join(d, "work")
+        os.makedirs(work)
+        st = np.loadtxt  # noqa: F841
+        import re
+        nb = int(re.search(r"Nbunch (\S+)", open(os.path.join(d, "params.txt")).read()).group(1)) \
+            if os.path.exists(os.path.join(d, "params.txt")) else None
+        if nb is None:
+            r = subprocess.run([HARNESS, conf, d, "1"], capture_output=True, text=True)
+            nb = int(float(re.search(r"^Nbunch (\S+)$", open(os.path.join(d, "params.txt")).read(), re.M).group(1)))
+        cmd = "ulimit -s unlimited; %s -usize %d -n %d %s %s %s/" % (MPIEXEC, nb + 1, nb + 1, MPI_BIN, conf, work)
+        r = subprocess.run(["bash", "-c", cmd], capture_output=True, text=True, cwd=d, timeout=600)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        if os.path.exists(out):
+            shutil.rmtree(out)
+        os.makedirs(out)
+        for f in sorted(os.listdir(work)):
+            if f.endswith(".log"):
+                continue
+            shutil.copy(os.path.join(work, f), os.path.join(out, f))
+        for f in ("Ldist", "Hdist", "Vdist"):
+            if os.path.exists(os.path.join(d, f)):
+                shutil.copy(os.path.join(d, f), os.path.join(out, f))
+        with open(os.path.join(out, "stdout.txt"), "w") as f:
+            f.write(re.sub(r"at time: [0-9:]+", "at time: T", r.stdout))
+    return out
+
+
 if __name__ == "__main__":
-    for n in (sys.argv[1:] or sorted(VARIANTS)):
-        p = make(n)
-        print(n, os.path.getsize(p))
+    names = sys.argv[1:] or sorted(VARIANTS) + ["e2e:wakes"]
+    for n in names:
+        if n.startswith("e2e:"):
+            print(n, make_e2e(n[4:]))
+        else:
+            p = make(n)
+            print(n, os.path.getsize(p))
