@@ -185,7 +185,7 @@ __device__ __forceinline__ void qe_normals(uint32_t seed, uint32_t bucket, unsig
 
 // Slice index of src/transform_weak.c:611-613 with x86-64 conversion semantics
 // (NaN and out-of-range values give INT_MIN, as cvttsd2si does).
-__device__ __forceinline__ int bin_of(double tau, double sgm, double Nsigma, double dT)
+__device__ __noinline__ int bin_of(double tau, double sgm, double Nsigma, double dT)
 {
   double Tau = -tau / sgm;
   double v = (Tau + Nsigma) / dT + 1;
@@ -262,16 +262,100 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
+// sin and cos of x for |x| < 2^19 pi/2: Cody-Waite reduction by pi/2 in three
+// parts (fdlibm's pio2_1, pio2_2, pio2_2t; the first two carry 33 bits, so k*part
+// is exact for |k| < 2^20), then Taylor polynomials to degree 17/18 on
+// [-pi/4, pi/4] (truncation < 1e-19), cos with the 1 - z/2 correction.  Larger
+// arguments fall back to the library sincos.
+__device__ __noinline__ double2 sincos_lib(double x)
+{
+  double2 r;
+  sincos(x, &r.x, &r.y);
+  return r;
+}
+
+__device__ __forceinline__ void sincos_cw(double x, double *s, double *c)
+{
+  if (!(fabs(x) < 823549.6)) {
+    const double2 r = sincos_lib(x);
+    *s = r.x;
+    *c = r.y;
+    return;
+  }
+  const double k = rint(x * 6.36619772367581382433e-01);
+  double r = fma(-k, 1.57079632673412561417e+00, x);
+  r = fma(-k, 6.07710050630396597660e-11, r);
+  r = fma(-k, 2.02226624879595063154e-21, r);
+  const double z = r * r;
+  double ps = fma(z, 2.8114572543455206e-15, -7.647163731819816e-13);
+  ps = fma(z, ps, 1.6059043836821613e-10);
+  ps = fma(z, ps, -2.505210838544172e-08);
+  ps = fma(z, ps, 2.7557319223985893e-06);
+  ps = fma(z, ps, -0.0001984126984126984);
+  ps = fma(z, ps, 0.008333333333333333);
+  ps = fma(z, ps, -0.16666666666666666);
+  const double sn = fma(r * z, ps, r);
+  double pc = fma(z, -1.5619206968586225e-16, 4.779477332387385e-14);
+  pc = fma(z, pc, -1.1470745597729725e-11);
+  pc = fma(z, pc, 2.08767569878681e-09);
+  pc = fma(z, pc, -2.755731922398589e-07);
+  pc = fma(z, pc, 2.48015873015873e-05);
+  pc = fma(z, pc, -0.001388888888888889);
+  pc = fma(z, pc, 0.041666666666666664);
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  const double cs = w + (((1.0 - w) - hz) + (z * z) * pc);
+  // quadrant without branches: swap on odd q, sin negative for q = 2,3, cos for q = 1,2
+  const int q = ((int)k) & 3;
+  const double a = (q & 1) ? cs : sn;
+  const double bq = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -bq : bq;
+}
+
+__device__ __forceinline__ double sin_cw(double x)
+{
+  double s, c;
+  sincos_cw(x, &s, &c);
+  return s;
+}
+
+// Slice index with reciprocal multiplies, falling back to the exact reference
+// expression (two correctly rounded divides) whenever the truncation could
+// depend on the last bits: the approximate value is within 1e-12 of the exact
+// one for |v| < 1e6.
+struct BinK {
+  double sgm, inv_sgm, Nsigma, dT, inv_dT;
+};
+__device__ __forceinline__ int bin_fast(double tau, const BinK &k)
+{
+  const double vf = fma(fma(-tau, k.inv_sgm, k.Nsigma), k.inv_dT, 1.0);
+  if (fabs(vf) < 1.0e6) {
+    const double lo = trunc(vf - 1.0e-9), hi = trunc(vf + 1.0e-9);
+    if (lo == hi) return (int)lo;
+  }
+  return bin_of(tau, k.sgm, k.Nsigma, k.dT);
+}
+
+// Block index -> work item, so that consecutive work items (the chunks of one
+// bunch) run on the same XCD (blocks are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ int xcd_remap(int bid, int n)
+{
+  const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
+template <int MODE, bool QE, bool DIP, bool GEN, bool MON>
 __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long long *wg_first, const long long *wg_end,
                                              const int *wg_bunch, const int *bucket, const double *shift,
                                              const double *sfL, const double *sfV, const double *sfH,
                                              const double *lrL, const double *lrV, const double *lrH,
                                              const double *pending, double *partial, HistPtr H,
-                                             const TurnState *ts, int mode)
+                                             const TurnState *ts)
 {
   extern __shared__ unsigned long long smem[];
   const int Nc = P.Nc;
-  // LDS layout: sx lo/hi [2*Nc], sz lo/hi [2*Nc], trash sums [8], counts [Nc] + trash [2], kick tables [3*Nc]
+  // LDS: sx lo/hi [2Nc], sz lo/hi [2Nc], trash sums [8], kick tables [3Nc], counts [Nc] + trash [2]
   unsigned long long *lsx = smem;
   unsigned long long *lsz = lsx + 2 * Nc;
   unsigned long long *ltr = lsz + 2 * Nc;
@@ -283,25 +367,25 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   __shared__ double red[WG / 64][NMOM];
   __shared__ int lbad;
 
-  const int wg = blockIdx.x;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const int b = wg_bunch[wg];
   const int kb = bucket[b];
   const long long j0 = wg_first[wg], j1 = wg_end[wg];
   const int tid = threadIdx.x;
   const long long rev = ts->rev;
-  const bool do_hist = Nc > 0 && mode != MODE_FINAL;
-  const bool monitor = mode == MODE_TURN && ((rev + 1) % P.monitor_every == 0);
+  const bool do_hist = Nc > 0 && MODE != MODE_FINAL;
 
-  for (int i = tid; i < 2 * Nc; i += WG) {
-    lsx[i] = 0;
-    lsz[i] = 0;
-  }
+  if (DIP)
+    for (int i = tid; i < 2 * Nc; i += WG) {
+      lsx[i] = 0;
+      lsz[i] = 0;
+    }
   for (int i = tid; i < Nc; i += WG) {
     lcnt[i] = 0;
-    if (mode == MODE_TURN) {
-      kL[i] = sfL[(size_t)b * Nc + i] + lrL[(size_t)kb * Nc + i];
-      kV[i] = sfV[(size_t)b * Nc + i] + lrV[(size_t)kb * Nc + i];
-      kH[i] = sfH[(size_t)b * Nc + i] + lrH[(size_t)kb * Nc + i];
+    if (MODE == MODE_TURN) {
+      if (P.kickL) kL[i] = sfL[(size_t)b * Nc + i] + lrL[(size_t)kb * Nc + i];
+      if (P.kickV) kV[i] = sfV[(size_t)b * Nc + i] + lrV[(size_t)kb * Nc + i];
+      if (P.kickH) kH[i] = sfH[(size_t)b * Nc + i] + lrH[(size_t)kb * Nc + i];
     }
   }
   if (tid < 8) ltr[tid] = 0;
@@ -309,82 +393,78 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   if (tid == 0) lbad = 0;
   __syncthreads();
 
+  const BinK bk = {P.sigma_tau, 1.0 / P.sigma_tau, P.Nsigma, P.dT, 1.0 / P.dT};
   double sh[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) sh[c] = shift[b * 6 + c];
-  const double rwx = (mode != MODE_STATS && P.rw && P.track_h) ? pending[2 * b] : 0.0;
-  const double rwz = (mode != MODE_STATS && P.rw && P.track_v) ? pending[2 * b + 1] : 0.0;
+  const bool rw_h = MODE != MODE_STATS && P.rw && P.track_h;
+  const bool rw_v = MODE != MODE_STATS && P.rw && P.track_v;
+  const double rwx = rw_h ? pending[2 * b] : 0.0;
+  const double rwz = rw_v ? pending[2 * b + 1] : 0.0;
   const double eps_const = ts->Vrf0 / P.E0k;
   const double phai0 = ts->phai0;
-  double ahc_ph[MBT_MAX_AHC];
-  for (int a = 0; a < P.n_ahc; a++) ahc_ph[a] = P.ahc_nHC[a] * kb * 2 * M_PI;
+  const long long jb0 = (long long)b * P.Np;
 
   double s1[6] = {0, 0, 0, 0, 0, 0}, s2[6] = {0, 0, 0, 0, 0, 0}, am[3] = {0, 0, 0};
 
   for (long long j = j0 + tid; j < j1; j += WG) {
     double tau = C.tau[j], del = C.del[j], x = C.x[j], xp = C.xp[j], z = C.z[j], zp = C.zp[j];
-    if (mode != MODE_STATS) {
-      if (P.rw) {
-        if (P.track_h) xp += rwx;
-        if (P.track_v) zp += rwz;
-      }
-    }
-    if (mode == MODE_TURN) {
+    if (rw_h) xp += rwx;
+    if (rw_v) zp += rwz;
+    if (MODE == MODE_TURN) {
       if (P.sf_active) {
-        int bn = bin_of(tau, P.sigma_tau, P.Nsigma, P.dT);
-        if (bn < 0) bn = 0;
-        else if (bn >= Nc) bn = Nc - 1;
+        int bn = bin_fast(tau, bk);
+        bn = bn < 0 ? 0 : (bn >= Nc ? Nc - 1 : bn);
         if (P.kickL) del += kL[bn];
         if (P.kickV) zp += kV[bn];
         if (P.kickH) xp += kH[bn];
       }
-      // ---- one-turn map, src/transform_weak.c:21-192
+      // ---- one-turn map, src/transform_weak.c:21-192 (operation order kept)
       const double tau0 = tau, del0 = del, x0 = x, xp0 = xp, z0 = z, zp0 = zp;
       double g;
-      if (!P.ideal_hc) {
-        g = eps_const * sin(P.wrf * tau0 + phai0);
-        for (int a = 0; a < P.n_ahc; a++) g += P.ahc_amp[a] * sin(P.ahc_wn[a] * tau0 + P.ahc_phi[a] + ahc_ph[a]);
+      if (!GEN) {
+        g = eps_const * sin_cw(P.wrf * tau0 + phai0);
+      } else if (!P.ideal_hc) {
+        g = eps_const * sin_cw(P.wrf * tau0 + phai0);
+        for (int a = 0; a < P.n_ahc; a++)
+          g += P.ahc_amp[a] * sin_cw(P.ahc_wn[a] * tau0 + P.ahc_phi[a] + P.ahc_nHC[a] * kb * 2 * M_PI);
       } else {
-        g = eps_const * (sin(P.wrf * tau0 + phai0) + P.HC_k * sin(P.m_wrf * tau0 + P.m_phi));
+        g = eps_const * (sin_cw(P.wrf * tau0 + phai0) + P.HC_k * sin_cw(P.m_wrf * tau0 + P.m_phi));
       }
       del = del0 + g - P.Urad;
       double nq[5], rff = 0.0;
-      if (P.qe) {
-        qe_normals(P.seed, (uint32_t)kb, (unsigned long long)rev, (uint32_t)(j - (long long)b * P.Np), nq);
-      }
-      if (P.qe) {
+      if (QE) {
+        qe_normals(P.seed, (uint32_t)kb, (unsigned long long)rev, (uint32_t)(j - jb0), nq);
         del = del * P.damp + P.excite * nq[0];
         rff = (1.0 + del0) / (1.0 + del0 + g);
       }
       tau = tau0 - del * P.T0 * P.ac;
       if (P.track_v) {
         double dQ = 0.0;
-        if (P.amp_dq) {
+        if (GEN && P.amp_dq) {
           double EV = P.gV * z0 * z0 + 2 * P.aV * z0 * zp0 + P.bV * zp0 * zp0;
           double EH = P.gV * x0 * x0 + 2 * P.aV * x0 * xp0 + P.bV * xp0 * xp0;
           dQ = P.CVH * EH + P.CVV * EV;
         }
-        double Psi = P.PsiV0 * (1.0 + P.Gziz * del0 + dQ);
         double sn, cs;
-        sincos(Psi, &sn, &cs);
+        sincos_cw(P.PsiV0 * (1.0 + P.Gziz * del0 + dQ), &sn, &cs);
         double a11 = cs + P.aV * sn, a12 = P.bV * sn, a21 = -P.gV * sn, a22 = cs - P.aV * sn;
         z = a11 * z0 + a12 * zp0;
         zp = a21 * z0 + a22 * zp0;
-        if (P.qe) {
+        if (QE) {
           z = z + P.qVp * nq[1];
           zp = rff * zp + P.qVs * nq[2];
         }
       }
       if (P.track_h) {
         double dQ = 0.0;
-        if (P.amp_dq) {
+        if (GEN && P.amp_dq) {
           double EV = P.gV * z0 * z0 + 2 * P.aV * z0 * zp0 + P.bV * zp0 * zp0;
           double EH = P.gV * x0 * x0 + 2 * P.aV * x0 * xp0 + P.bV * xp0 * xp0;
           dQ = P.CHH * EH + P.CHV * EV;
         }
-        double Psi = P.PsiH0 * (1.0 + P.Gzix * del0 + dQ);
         double sn, cs;
-        sincos(Psi, &sn, &cs);
+        sincos_cw(P.PsiH0 * (1.0 + P.Gzix * del0 + dQ), &sn, &cs);
         double a11 = cs + P.aH * sn;
         double a12 = P.bH * sn;
         double a13 = (1.0 - a11) * P.dispH - a12 * P.disppH;
@@ -393,13 +473,13 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
         double a23 = -a21 * P.dispH + (1.0 - a22) * P.disppH;
         x = a11 * x0 + a12 * xp0 + a13 * del;
         xp = a21 * x0 + a22 * xp0 + a23 * del;
-        if (P.qe) {
+        if (QE) {
           x = x + P.qHp * nq[3];
           xp = rff * xp + P.qHs * nq[4];
         }
       }
     }
-    if (mode != MODE_STATS) {
+    if (MODE != MODE_STATS) {
       C.tau[j] = tau;
       C.del[j] = del;
       C.x[j] = x;
@@ -407,45 +487,47 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
       C.z[j] = z;
       C.zp[j] = zp;
     }
-    // ---- moments about the per-bunch shift
+    // ---- moments about the per-bunch shift (reassociated: no bitwise target)
     const double v[6] = {tau, del, x, xp, z, zp};
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      double d = v[c] - sh[c];
+      const double d = v[c] - sh[c];
       s1[c] += d;
-      s2[c] += d * d;
+      s2[c] = fma(d, d, s2[c]);
     }
-    if (monitor) {
+    if (MON) {
       am[0] += P.gamma1[0] * (tau * tau) + 2.0 * P.alpha1[0] * tau * del + P.beta1[0] * (del * del);
       am[1] += P.gamma1[1] * (x * x) + 2.0 * P.alpha1[1] * x * xp + P.beta1[1] * (xp * xp);
       am[2] += P.gamma1[2] * (z * z) + 2.0 * P.alpha1[2] * z * zp + P.beta1[2] * (zp * zp);
     }
     // ---- next turn's slice histogram
     if (do_hist) {
-      int bn = bin_of(tau, P.sigma_tau, P.Nsigma, P.dT);
-      unsigned long long xl, xh, zl, zh;
-      bool big = !(fabs(x) < 1048576.0) || !(fabs(z) < 1048576.0);
-      if (big) {
-        lbad = 1;
-        xl = xh = zl = zh = 0;
-      } else {
-        fx_split(x, xl, xh);
-        fx_split(z, zl, zh);
-      }
-      if (bn >= 0 && bn < Nc) {
-        atomicAdd(&lcnt[bn], 1u);
-        fx_atomic_add(&lsx[2 * bn], &lsx[2 * bn + 1], xl, xh);
-        fx_atomic_add(&lsz[2 * bn], &lsz[2 * bn + 1], zl, zh);
-      } else {
-        int t = bn < 0 ? 0 : 1;
-        atomicAdd(&ltc[t], 1u);
-        fx_atomic_add(&ltr[4 * t], &ltr[4 * t + 1], xl, xh);
-        fx_atomic_add(&ltr[4 * t + 2], &ltr[4 * t + 3], zl, zh);
+      const int bn = bin_fast(tau, bk);
+      const bool inr = bn >= 0 && bn < Nc;
+      if (inr) atomicAdd(&lcnt[bn], 1u);
+      else atomicAdd(&ltc[bn < 0 ? 0 : 1], 1u);
+      if (DIP) {
+        unsigned long long xl, xh, zl, zh;
+        if (!(fabs(x) < 1048576.0) || !(fabs(z) < 1048576.0)) {
+          lbad = 1;
+          xl = xh = zl = zh = 0;
+        } else {
+          fx_split(x, xl, xh);
+          fx_split(z, zl, zh);
+        }
+        if (inr) {
+          fx_atomic_add(&lsx[2 * bn], &lsx[2 * bn + 1], xl, xh);
+          fx_atomic_add(&lsz[2 * bn], &lsz[2 * bn + 1], zl, zh);
+        } else {
+          const int t = bn < 0 ? 0 : 1;
+          fx_atomic_add(&ltr[4 * t], &ltr[4 * t + 1], xl, xh);
+          fx_atomic_add(&ltr[4 * t + 2], &ltr[4 * t + 3], zl, zh);
+        }
       }
     }
   }
 
-  // ---- block reduction of the 15 moment sums in a fixed order
+  // ---- block reduction of the moment sums in a fixed order
   double vals[NMOM];
 #pragma unroll
   for (int c = 0; c < 6; c++) {
@@ -458,32 +540,38 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int c = 0; c < NMOM; c++) {
-    double s = wave_sum(vals[c]);
-    if (lane == 0) red[wv][c] = s;
+    if (!MON && c >= 12) continue;
+    double sum = wave_sum(vals[c]);
+    if (lane == 0) red[wv][c] = sum;
   }
   __syncthreads();
   if (tid < NMOM) {
-    double s = 0.0;
-    for (int w = 0; w < WG / 64; w++) s += red[w][tid];
-    partial[(size_t)wg * NMOM + tid] = s;
+    double sum = 0.0;
+    if (MON || tid < 12)
+      for (int w = 0; w < WG / 64; w++) sum += red[w][tid];
+    partial[(size_t)wg * NMOM + tid] = sum;
   }
   // ---- flush the block histogram with exact integer atomics
   if (do_hist) {
     for (int i = tid; i < Nc; i += WG) {
-      unsigned int n = lcnt[i];
+      const unsigned int n = lcnt[i];
       if (n == 0) continue;
-      size_t g = (size_t)b * Nc + i;
-      atomicAdd(&H.cnt[g], n);
-      fx_atomic_add(&H.sx[2 * g], &H.sx[2 * g + 1], lsx[2 * i], lsx[2 * i + 1]);
-      fx_atomic_add(&H.sz[2 * g], &H.sz[2 * g + 1], lsz[2 * i], lsz[2 * i + 1]);
+      const size_t gi = (size_t)b * Nc + i;
+      atomicAdd(&H.cnt[gi], n);
+      if (DIP) {
+        fx_atomic_add(&H.sx[2 * gi], &H.sx[2 * gi + 1], lsx[2 * i], lsx[2 * i + 1]);
+        fx_atomic_add(&H.sz[2 * gi], &H.sz[2 * gi + 1], lsz[2 * i], lsz[2 * i + 1]);
+      }
     }
     if (tid < 2 && ltc[tid] > 0) {
-      int t = tid;
+      const int t = tid;
       atomicAdd(&H.tcnt[2 * b + t], ltc[t]);
-      fx_atomic_add(&H.tsx[4 * b + 2 * t], &H.tsx[4 * b + 2 * t + 1], ltr[4 * t], ltr[4 * t + 1]);
-      fx_atomic_add(&H.tsz[4 * b + 2 * t], &H.tsz[4 * b + 2 * t + 1], ltr[4 * t + 2], ltr[4 * t + 3]);
+      if (DIP) {
+        fx_atomic_add(&H.tsx[4 * b + 2 * t], &H.tsx[4 * b + 2 * t + 1], ltr[4 * t], ltr[4 * t + 1]);
+        fx_atomic_add(&H.tsz[4 * b + 2 * t], &H.tsz[4 * b + 2 * t + 1], ltr[4 * t + 2], ltr[4 * t + 3]);
+      }
     }
-    if (tid == 0 && lbad) H.bad[b] = 1;
+    if (DIP && tid == 0 && lbad) H.bad[b] = 1;
   }
 }
 
@@ -725,19 +813,29 @@ __global__ __launch_bounds__(WG) void k_rw(DevParams P, const double *rwhist, co
   __shared__ double red[WG / 64][2];
   const int b = blockIdx.x, tid = threadIdx.x, kb = bucket[b];
   const long long rev = ts->rev;
-  const long long n = (long long)(P.Nmlt + 2) * P.h;
-  const long long M = (long long)(P.Nmlt + 1) * P.h;
-  const long long bpos = -rev * P.h + kb;
+  const int h = P.h;
+  const long long n = (long long)(P.Nmlt + 2) * h;
+  const long long M = (long long)(P.Nmlt + 1) * h;
+  // index of tap m in the cyclic history: (-rev h + kb + m) mod n, bucket (kb + m) mod h,
+  // advanced incrementally (stride WG) instead of by 64-bit divisions per tap
+  const long long m0 = 1 + tid;
+  long long k0 = (-rev * h + kb + m0) % n;
+  long long idx = k0 < 0 ? k0 + n : k0;
+  int in_mod = (int)((kb + m0) % h);
+  const long long step_n = WG % n;
+  const int step_h = WG % h;
   double ax = 0.0, az = 0.0;
-  for (long long m = 1 + tid; m <= M; m += WG) {
-    long long in_mod = (kb + m) % P.h;
-    if (!nfFill[in_mod]) continue;
-    long long k = bpos + m;
-    long long idx = ((k % n) + n) % n;
-    double I = Ibunch[in_mod];
-    double st = sqrt_tt[m];
-    ax += P.fNp * rwhist[2 * idx] * I / st;
-    az += P.fNp * rwhist[2 * idx + 1] * I / st;
+  for (long long m = m0; m <= M; m += WG) {
+    if (nfFill[in_mod]) {
+      const double I = Ibunch[in_mod];
+      const double st = sqrt_tt[m];
+      ax += P.fNp * rwhist[2 * idx] * I / st;
+      az += P.fNp * rwhist[2 * idx + 1] * I / st;
+    }
+    idx += step_n;
+    if (idx >= n) idx -= n;
+    in_mod += step_h;
+    if (in_mod >= h) in_mod -= h;
   }
   ax = wave_sum(ax);
   az = wave_sum(az);
@@ -1377,13 +1475,36 @@ extern "C" int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *s)
   return MBT_OK;
 }
 
+typedef void (*turn_kernel_t)(DevParams, Coords, const long long *, const long long *, const int *, const int *,
+                              const double *, const double *, const double *, const double *, const double *,
+                              const double *, const double *, const double *, double *, HistPtr, const TurnState *);
+
+#define TK(Q, D, G, M) k_turn<MODE_TURN, Q, D, G, M>
+static const turn_kernel_t g_turn_kernels[16] = {
+    TK(false, false, false, false), TK(false, false, false, true), TK(false, false, true, false),
+    TK(false, false, true, true),   TK(false, true, false, false), TK(false, true, false, true),
+    TK(false, true, true, false),   TK(false, true, true, true),   TK(true, false, false, false),
+    TK(true, false, false, true),   TK(true, false, true, false),  TK(true, false, true, true),
+    TK(true, true, false, false),   TK(true, true, false, true),   TK(true, true, true, false),
+    TK(true, true, true, true)};
+#undef TK
+
 static int launch_turn_pass(mbt_ring *r, int mode)
 {
-  hipLaunchKernelGGL(k_turn, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
+  turn_kernel_t k;
+  if (mode == MODE_STATS) k = k_turn<MODE_STATS, false, true, true, false>;
+  else if (mode == MODE_FINAL) k = k_turn<MODE_FINAL, false, true, true, false>;
+  else {
+    const bool mon = ((r->rev_host + 1) % r->cfg.NrevMon) == 0;
+    const bool dip = r->P.kickV || r->P.kickH || r->lr_plane[1] || r->lr_plane[2];
+    const bool gen = r->P.ideal_hc || r->P.n_ahc > 0 || r->P.amp_dq;
+    k = g_turn_kernels[(r->P.qe ? 8 : 0) + (dip ? 4 : 0) + (gen ? 2 : 0) + (mon ? 1 : 0)];
+  }
+  const size_t NcA = r->Nc > 0 ? r->Nc : 1;
+  hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
                      r->wg_bunch, r->bucket, r->shift, r->sfL, r->sfV, r->sfH, r->lr_wake,
-                     r->lr_wake + (size_t)r->h * (r->Nc > 0 ? r->Nc : 1),
-                     r->lr_wake + (size_t)2 * r->h * (r->Nc > 0 ? r->Nc : 1), r->pending, r->partial, r->H, r->ts,
-                     mode);
+                     r->lr_wake + (size_t)r->h * NcA, r->lr_wake + (size_t)2 * r->h * NcA, r->pending, r->partial,
+                     r->H, r->ts);
   CK(hipGetLastError());
   return MBT_OK;
 }
