@@ -190,6 +190,8 @@ typedef struct mbt_ring mbt_ring;
 
 /* flags */
 #define MBT_RING_NO_INIT_DIST 1 /* leave coordinates zero; caller uploads them */
+#define MBT_RING_NO_GRAPHS 2    /* launch every kernel of every turn eagerly instead of replaying
+                                   HIP graphs of whole blocks of turns */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);
@@ -225,6 +227,21 @@ double mbt_ring_bytes_per_turn(const mbt_ring *r);
 /* Per-kernel timing with HIP events around every launch group (0 = off). */
 int mbt_ring_set_timing(mbt_ring *r, int on);
 long mbt_ring_turn(const mbt_ring *r);
+
+/* ---- ensembles of independent rings on one GPU ------------------------------ */
+/* No counterpart in the reference, which runs one ring per MPI job: nrings
+ * independent configurations (their own bunch counts, macroparticle counts,
+ * currents, optics, wakes and RF feedback) tracked by the same kernels, every
+ * launch covering all rings.  A single ring is an ensemble of one, and every
+ * mbt_ring_* call above accepts an ensemble: bunch ordinals, stats and ampinv rows
+ * run over the rings' bunches in ring order (ring 0 first), rf rows hold
+ * nrings (phai0, Vrf0) pairs.  The rings must share Ncell and NrevMon. */
+int mbt_ensemble_create(const mbt_config *const *cfgs, int nrings, int device, int flags, mbt_ring **out);
+/* Coordinates of global bunch ordinal g (the ring's own Np, SoA as mbt_generate_bunch). */
+int mbt_ensemble_set_bunch(mbt_ring *r, int g, const double *soa6);
+int mbt_ensemble_get_bunch(const mbt_ring *r, int g, double *soa6);
+/* Number of rings, total bunches and total macroparticles. */
+int mbt_ensemble_size(const mbt_ring *r, int *nrings, int *nbunch_total, long long *nparticles);
 
 /* ---- end-to-end drop-in ----------------------------------------------------- */
 /* Runs manager_weakweak + all workers of the reference on one GPU and writes the

@@ -11,12 +11,13 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libmbtrack_mi355x.so")
+LIBPATH = os.environ.get("MBTRACK_LIB") or os.path.join(HERE, "libmbtrack_mi355x.so")
 EXE = os.path.join(HERE, "mbtrack-mi355x")
 
 NSTAT = 12
 NAMP = 6
 MBT_RING_NO_INIT_DIST = 1
+MBT_RING_NO_GRAPHS = 2
 
 _lib = None
 
@@ -57,6 +58,10 @@ def lib():
         L.mbt_ring_set_timing.argtypes = [vp, ip]
         L.mbt_ring_turn.argtypes = [vp]
         L.mbt_ring_turn.restype = ctypes.c_long
+        L.mbt_ensemble_create.argtypes = [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(vp)]
+        L.mbt_ensemble_set_bunch.argtypes = [vp, ip, dp]
+        L.mbt_ensemble_get_bunch.argtypes = [vp, ip, dp]
+        L.mbt_ensemble_size.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_longlong)]
         if hasattr(L, "mbt_manager_weakweak"):
             L.mbt_manager_weakweak.argtypes = [vp, ip]
         if hasattr(L, "mbt_wofz"):
@@ -68,6 +73,10 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise MbtError("mbtrack error %d: %s" % (rc, lib().mbt_last_error().decode()))
+
+
+def _flags(init_dist, graphs):
+    return (0 if init_dist else MBT_RING_NO_INIT_DIST) | (0 if graphs else MBT_RING_NO_GRAPHS)
 
 
 def dptr(a):
@@ -123,14 +132,14 @@ class Config:
 class Ring:
     """All bunches of one ring resident on one GPU (mbt_ring_* C ABI)."""
 
-    def __init__(self, cfg, device=0, init_dist=True):
+    def __init__(self, cfg, device=0, init_dist=True, graphs=True):
         L = lib()
         self.cfg = cfg
         self.Nb = cfg.geti("Nbunch")
         self.Np = cfg.geti("Np")
         self.Nc = cfg.geti("Ncell")
         p = ctypes.c_void_p()
-        _check(L.mbt_ring_create(cfg.ptr, device, 0 if init_dist else MBT_RING_NO_INIT_DIST, ctypes.byref(p)))
+        _check(L.mbt_ring_create(cfg.ptr, device, _flags(init_dist, graphs), ctypes.byref(p)))
         self.ptr = p
 
     def close(self):
@@ -183,3 +192,52 @@ class Ring:
 
     def bytes_per_turn(self):
         return lib().mbt_ring_bytes_per_turn(self.ptr)
+
+
+class Ensemble(Ring):
+    """Independent rings tracked together on one GPU (mbt_ensemble_* C ABI).
+
+    Bunch ordinals, stats and ampinv rows run over the rings' bunches in ring
+    order; rf rows hold one (phai0, Vrf0) pair per ring."""
+
+    def __init__(self, cfgs, device=0, init_dist=True, graphs=True):
+        L = lib()
+        self.cfgs = list(cfgs)
+        self.R = len(self.cfgs)
+        self.nb = [c.geti("Nbunch") for c in self.cfgs]
+        self.nps = [c.geti("Np") for c in self.cfgs]
+        self.Nb = sum(self.nb)
+        self.Nc = self.cfgs[0].geti("Ncell") if self.cfgs else 0
+        self.bunch_np = np.repeat(self.nps, self.nb)
+        self.first = np.concatenate([[0], np.cumsum(self.nb)])
+        arr = (ctypes.c_void_p * self.R)(*[c.ptr.value for c in self.cfgs])
+        p = ctypes.c_void_p()
+        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs), ctypes.byref(p)))
+        self.ptr = p
+
+    def size(self):
+        r, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().mbt_ensemble_size(self.ptr, ctypes.byref(r), ctypes.byref(g), ctypes.byref(n)))
+        return r.value, g.value, n.value
+
+    def ring_rows(self, k, arr):
+        """Rows of ring k in a per-bunch array [..., G, ...] (bunch axis -2)."""
+        return arr[..., self.first[k]:self.first[k + 1], :]
+
+    def set_bunch(self, g, soa6):
+        a = np.ascontiguousarray(soa6, dtype=np.float64).reshape(-1)
+        if a.size != 6 * self.bunch_np[g]:
+            raise MbtError("bunch %d holds %d macroparticles" % (g, self.bunch_np[g]))
+        _check(lib().mbt_ensemble_set_bunch(self.ptr, g, dptr(a)))
+
+    def get_bunch(self, g):
+        a = np.zeros(6 * int(self.bunch_np[g]))
+        _check(lib().mbt_ensemble_get_bunch(self.ptr, g, dptr(a)))
+        return a.reshape(6, -1)
+
+    def track(self, nturns, want_stats=True, want_amp=True, want_rf=True):
+        st = np.zeros((nturns, self.Nb, NSTAT)) if want_stats else None
+        am = np.zeros((nturns, self.Nb, NAMP)) if want_amp else None
+        rf = np.zeros((nturns, self.R, 2)) if want_rf else None
+        _check(lib().mbt_ring_track(self.ptr, nturns, dptr(st), dptr(am), dptr(rf)))
+        return st, am, rf

@@ -112,3 +112,47 @@ def wl_production(turns=20000, np_=100000, rw=True, bbr=True):
             "distribution": {"Np": np_},
         },
         extra_sections=extra)
+
+
+# ---- batched ensemble of independent rings ------------------------------------
+
+def wl_ensemble(d, nrings=256, seed=20240607, turns=10000, np_range=(10000, 100000), nb_range=(32, 416),
+                ib_range=(0.1, 5.0), chrom_range=(-1.0, 3.0)):
+    """Writes nrings independent ring configurations into directory d and
+    returns their paths.  Each ring: the benchmark machine (h = 416), a bunch
+    count and macroparticle count drawn log-uniform from nb_range / np_range,
+    a bunch current uniform in ib_range [mA] and horizontal and vertical
+    chromaticities uniform in chrom_range, equally spaced bunches, 100 slices,
+    broadband resonator (10 GHz, Q 1, 5 kOhm), resistive wall over 50 past
+    turns, the main cavity as a long-range resonator with RF beam-loading
+    feedback, quantum excitation off.  Draws come from numpy's PCG64 with
+    the given seed."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    os.makedirs(d, exist_ok=True)
+    paths = []
+    h = 416
+    for k in range(nrings):
+        nb = int(round(np.exp(rng.uniform(np.log(nb_range[0]), np.log(nb_range[1])))))
+        npb = int(round(np.exp(rng.uniform(np.log(np_range[0]), np.log(np_range[1])))))
+        ib = float(rng.uniform(*ib_range))
+        cx, cz = (float(v) for v in rng.uniform(chrom_range[0], chrom_range[1], 2))
+        nb = max(1, min(nb, h))
+        fill = os.path.join(d, "fill_%03d.txt" % k)
+        filling_file(fill, [int(i * h / nb) for i in range(nb)], out=())
+        text = ring_conf(
+            sections={
+                "selffield": {"Ncell": 100, "Nsigma": 5.0},
+                "tracking": {"NrevTot": turns, "NrevMon": 100, "EnableRW_long": 1, "Nmlt": 50,
+                             "EnableQuantum": 0, "filling": fill},
+                "ring": {"Iring": ib * nb, "Cx": cx, "Cz": cz, "rhorw": 1.7e-8, "beffL1": 0.012,
+                         "beffL2": 0.012, "beffH3": 0.012, "beffV3": 0.012},
+                "distribution": {"Np": npb, "iseed_LON": 1001 + 3 * k, "iseed_HOR": 1002 + 3 * k,
+                                 "iseed_VER": 1003 + 3 * k},
+            },
+            extra_sections=[("resonator_1", {"Rs": 5000.0, "fres": 10.0, "Qfactor": 1.0, "plane": "LON"}),
+                            ("harmonic_cavity_1", {"m": 1, "Rs": 1e6, "detune": -40e3, "Qfactor": 4e4}),
+                            ("rf_feedback", {"resonator": 1, "averaging_length": 416})])
+        paths.append(write(os.path.join(d, "ring_%03d.conf" % k), text))
+    return paths

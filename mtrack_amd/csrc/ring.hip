@@ -1,7 +1,11 @@
 // The multi-bunch tracking turn on one MI355X (gfx950).
 //
 // Every bunch is resident in HBM as six FP64 coordinate arrays (tau, delta, x,
-// x', z, z'), bunches back to back.  One turn is four kernels on one stream:
+// x', z, z'), bunches back to back.  The unit is an ensemble of independent
+// rings (one ring is an ensemble of one): per-ring constants sit in a DevParams
+// table indexed through bunch -> ring, so every launch covers all rings and
+// rings of uneven bunch and macroparticle counts share each kernel.  One turn
+// is four kernels on one stream:
 //
 //   k_lr      long-range resonator phasor recurrence over all buckets and the
 //             RF-feedback solve (construct_wake_phasor src/transform_weak.c:961-1048,
@@ -32,8 +36,13 @@
 #pragma clang fp contract(off)
 
 #define WG 256
+#ifndef CHUNK_ITEMS
 #define CHUNK_ITEMS 8
+#endif
 #define NMOM 15
+#ifndef TURN_WAVES_PER_EU
+#define TURN_WAVES_PER_EU 4
+#endif
 #define STATS_TURNS 256
 #define MAXLR (3 * MBT_MAX_LR)
 
@@ -46,20 +55,25 @@
     }                                                                                            \
   } while (0)
 
+// Per-ring constants of the turn (one entry per ring of an ensemble; a single
+// ring is an ensemble of one).  Derived on the host with glibc libm from the
+// reference's formulas so that every constant equals the reference's.
 struct DevParams {
-  int Nb, h, Nc, Nmlt;
+  int ring, Nb, h, Nc, Nmlt;
+  int bunch_off;  // first global bunch of the ring
+  int bucket_off; // first global bucket of the ring
+  long long rw_off, st_off; // offsets into the RW history and sqrt table
   int track_h, track_v;
   int qe, ideal_hc, amp_dq;
   int kickL, kickV, kickH; // selffield kick of the plane is applied
   int convL;               // longitudinal Green function present (PlaneL > 0)
-  int scan;
-  long long NrevScan;
-  double scan_step;
-  long long Np;
   int sf_active;
   int rw;
   int n_ahc;
   int monitor_every;
+  int scan;
+  long long NrevScan;
+  double scan_step;
   unsigned seed;
   int m_aHC;
   double ahc_amp[MBT_MAX_AHC], ahc_wn[MBT_MAX_AHC], ahc_phi[MBT_MAX_AHC], ahc_nHC[MBT_MAX_AHC];
@@ -74,6 +88,10 @@ struct DevParams {
   double gamma1[3], alpha1[3], beta1[3];
   double fNp;
   double rwconst[2][2]; // [plane HOR/VER][trigger off/on]
+  // RF feedback (src/feedback_rf.c, src/input.c:1678-1690)
+  int fb_len, fb_switch_off;
+  long long fbh_off;
+  double vrf_design, phi0_design;
 };
 
 struct TurnState {
@@ -87,7 +105,10 @@ struct TurnState {
 };
 
 struct LRConst {
-  int plane, l, fb;
+  int ring, plane, l, fb;
+  int h, bucket_off;
+  int init_turns; // wake_phasor_init replay length Nbumax / h
+  long long lrc_off;
   double pr0, pr1;   // one bin
   double ge, gc, gs; // bucket gap
   double ee, ec, es; // empty bucket
@@ -346,14 +367,22 @@ __device__ __forceinline__ int xcd_remap(int bid, int n)
 }
 
 template <int MODE, bool QE, bool DIP, bool GEN, bool MON>
-__global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long long *wg_first, const long long *wg_end,
-                                             const int *wg_bunch, const int *bucket, const double *shift,
-                                             const double *sfL, const double *sfV, const double *sfH,
-                                             const double *lrL, const double *lrV, const double *lrH,
-                                             const double *pending, double *partial, HistPtr H,
-                                             const TurnState *ts)
+__global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams *__restrict__ Pr, Coords C,
+                                                                const long long *wg_first, const long long *wg_end,
+                                                                const int *wg_bunch, const int *bunch_ring,
+                                                                const int *bucket, const long long *bunch_first,
+                                                                const double *shift, const double *sfL,
+                                                                const double *sfV, const double *sfH,
+                                                                const double *lr_wake, long long lr_plane_stride,
+                                                                const double *pending, double *partial, HistPtr H,
+                                                                const TurnState *__restrict__ tsr)
 {
   extern __shared__ unsigned long long smem[];
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = wg_bunch[wg];
+  const int ring = bunch_ring[b];
+  const DevParams &P = Pr[ring];
+  const TurnState *ts = tsr + ring;
   const int Nc = P.Nc;
   // LDS: sx lo/hi [2Nc], sz lo/hi [2Nc], trash sums [8], kick tables [3Nc], counts [Nc] + trash [2]
   unsigned long long *lsx = smem;
@@ -367,13 +396,14 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   __shared__ double red[WG / 64][NMOM];
   __shared__ int lbad;
 
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int b = wg_bunch[wg];
   const int kb = bucket[b];
   const long long j0 = wg_first[wg], j1 = wg_end[wg];
   const int tid = threadIdx.x;
   const long long rev = ts->rev;
   const bool do_hist = Nc > 0 && MODE != MODE_FINAL;
+  const double *lrL = lr_wake + (size_t)(P.bucket_off + kb) * Nc;
+  const double *lrH = lrL + lr_plane_stride;
+  const double *lrV = lrH + lr_plane_stride;
 
   if (DIP)
     for (int i = tid; i < 2 * Nc; i += WG) {
@@ -383,9 +413,9 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   for (int i = tid; i < Nc; i += WG) {
     lcnt[i] = 0;
     if (MODE == MODE_TURN) {
-      if (P.kickL) kL[i] = sfL[(size_t)b * Nc + i] + lrL[(size_t)kb * Nc + i];
-      if (P.kickV) kV[i] = sfV[(size_t)b * Nc + i] + lrV[(size_t)kb * Nc + i];
-      if (P.kickH) kH[i] = sfH[(size_t)b * Nc + i] + lrH[(size_t)kb * Nc + i];
+      if (P.kickL) kL[i] = sfL[(size_t)b * Nc + i] + lrL[i];
+      if (P.kickV) kV[i] = sfV[(size_t)b * Nc + i] + lrV[i];
+      if (P.kickH) kH[i] = sfH[(size_t)b * Nc + i] + lrH[i];
     }
   }
   if (tid < 8) ltr[tid] = 0;
@@ -403,12 +433,23 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
   const double rwz = rw_v ? pending[2 * b + 1] : 0.0;
   const double eps_const = ts->Vrf0 / P.E0k;
   const double phai0 = ts->phai0;
-  const long long jb0 = (long long)b * P.Np;
+  const bool qe = QE && P.qe;
 
   double s1[6] = {0, 0, 0, 0, 0, 0}, s2[6] = {0, 0, 0, 0, 0, 0}, am[3] = {0, 0, 0};
 
-  for (long long j = j0 + tid; j < j1; j += WG) {
-    double tau = C.tau[j], del = C.del[j], x = C.x[j], xp = C.xp[j], z = C.z[j], zp = C.zp[j];
+  // block-uniform base pointers + 32-bit lane offsets: one offset register
+  // addresses all six arrays (scalar-base global loads)
+  double *__restrict__ pt = C.tau + j0;
+  double *__restrict__ pd = C.del + j0;
+  double *__restrict__ px = C.x + j0;
+  double *__restrict__ pxp = C.xp + j0;
+  double *__restrict__ pz = C.z + j0;
+  double *__restrict__ pzp = C.zp + j0;
+  const unsigned cnt = (unsigned)(j1 - j0);
+  const unsigned jloc0 = (unsigned)(j0 - bunch_first[b]);
+
+  for (unsigned i = tid; i < cnt; i += WG) {
+    double tau = pt[i], del = pd[i], x = px[i], xp = pxp[i], z = pz[i], zp = pzp[i];
     if (rw_h) xp += rwx;
     if (rw_v) zp += rwz;
     if (MODE == MODE_TURN) {
@@ -432,9 +473,9 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
         g = eps_const * (sin_cw(P.wrf * tau0 + phai0) + P.HC_k * sin_cw(P.m_wrf * tau0 + P.m_phi));
       }
       del = del0 + g - P.Urad;
-      double nq[5], rff = 0.0;
-      if (QE) {
-        qe_normals(P.seed, (uint32_t)kb, (unsigned long long)rev, (uint32_t)(j - jb0), nq);
+      double nq[5] = {0, 0, 0, 0, 0}, rff = 0.0;
+      if (qe) {
+        qe_normals(P.seed, (uint32_t)kb, (unsigned long long)rev, jloc0 + i, nq);
         del = del * P.damp + P.excite * nq[0];
         rff = (1.0 + del0) / (1.0 + del0 + g);
       }
@@ -451,7 +492,7 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
         double a11 = cs + P.aV * sn, a12 = P.bV * sn, a21 = -P.gV * sn, a22 = cs - P.aV * sn;
         z = a11 * z0 + a12 * zp0;
         zp = a21 * z0 + a22 * zp0;
-        if (QE) {
+        if (qe) {
           z = z + P.qVp * nq[1];
           zp = rff * zp + P.qVs * nq[2];
         }
@@ -473,19 +514,19 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
         double a23 = -a21 * P.dispH + (1.0 - a22) * P.disppH;
         x = a11 * x0 + a12 * xp0 + a13 * del;
         xp = a21 * x0 + a22 * xp0 + a23 * del;
-        if (QE) {
+        if (qe) {
           x = x + P.qHp * nq[3];
           xp = rff * xp + P.qHs * nq[4];
         }
       }
     }
     if (MODE != MODE_STATS) {
-      C.tau[j] = tau;
-      C.del[j] = del;
-      C.x[j] = x;
-      C.xp[j] = xp;
-      C.z[j] = z;
-      C.zp[j] = zp;
+      pt[i] = tau;
+      pd[i] = del;
+      px[i] = x;
+      pxp[i] = xp;
+      pz[i] = z;
+      pzp[i] = zp;
     }
     // ---- moments about the per-bunch shift (reassociated: no bitwise target)
     const double v[6] = {tau, del, x, xp, z, zp};
@@ -578,34 +619,43 @@ __global__ __launch_bounds__(WG) void k_turn(DevParams P, Coords C, const long l
 // ---------------------------------------------------------------- k_reduce
 
 struct RedOut {
-  double *stats;     // [STATS_TURNS][Nb][12]
-  double *amp;       // [STATS_TURNS][Nb][6]
-  double *shift;     // [Nb][6]
-  double *rwhist;    // [(Nmlt+2)h][2]
-  double *fnp_ring;  // [3][h*Nc]
+  double *stats;     // [STATS_TURNS][G][12]
+  double *amp;       // [STATS_TURNS][G][6]
+  double *shift;     // [G][6]
+  double *rwhist;    // per ring [(Nmlt+2)h][2] at rw_off
+  double *fnp_ring;  // [3][H*Nc]
+  long long plane_stride; // H*Nc
   double *sfL, *sfV, *sfH;
-  double *fnp_out, *dipV_out, *dipH_out; // last clamped histograms [Nb*Nc]
-  int *trash_tot;    // [Nb*2] cumulative (consumed)
-  int *trash_pend;   // [Nb*2]
-  const double *Gl, *Gv, *Gh;
+  double *fnp_out, *dipV_out, *dipH_out; // last clamped histograms [G*Nc]
+  int *trash_tot;    // [G*2] cumulative (consumed)
+  int *trash_pend;   // [G*2]
+  const double *G;   // Green functions [R][3][Nc]: Gl, Gv, Gh
   const double *bunch_Ib;
   const int *np;
+  int Gtot;
+  double *rfbuf;     // [STATS_TURNS][R][2] phase and voltage the turn's map used
+  int R;
 };
 
 // phase: 0 = means only (shift <- mean), 1 = initial statistics + RW history
 // fill, 2 = turn, 3 = final statistics
-__global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partial, const int *bunch_wg0, const int *bucket,
-                                               HistPtr H, RedOut R, const TurnState *ts, int phase, int lr_L,
-                                               int lr_H, int lr_V)
+__global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr, const double *partial,
+                                               const int *bunch_wg0, const int *bunch_ring, const int *bucket,
+                                               HistPtr H, RedOut R, const TurnState *__restrict__ tsr, int phase,
+                                               const int *lr_planes)
 {
   extern __shared__ double sm[];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int ring = bunch_ring[b];
+  const DevParams &P = Pr[ring];
+  const TurnState *ts = tsr + ring;
+  const int kb = bucket[b];
   const int Nc = P.Nc;
   double *fnp = sm;
   double *dV = fnp + Nc;
   double *dH = dV + Nc;
   __shared__ double mom[NMOM];
   __shared__ int icmin_s, icmax_s;
-  const int b = blockIdx.x, tid = threadIdx.x, kb = bucket[b];
   const long long rev = ts->rev;
   const int w0 = bunch_wg0[b], w1 = bunch_wg0[b + 1];
   const double Np = (double)R.np[b];
@@ -635,14 +685,19 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
         R.shift[b * 6 + 2 * pl + 1] = st[6 + pl];
       }
     } else {
-      double *o = R.stats + ((size_t)slot * P.Nb + b) * 12;
+      double *o = R.stats + ((size_t)slot * R.Gtot + b) * 12;
       for (int i = 0; i < 12; i++) o[i] = st[i];
       for (int pl = 0; pl < 3; pl++) {
         R.shift[b * 6 + 2 * pl] = st[pl];
         R.shift[b * 6 + 2 * pl + 1] = st[6 + pl];
       }
+      if (phase == 2 && b == P.bunch_off) {
+        // nothing changes phai0 / Vrf0 between the map and this point of the turn
+        R.rfbuf[((size_t)slot * R.R + ring) * 2] = ts->phai0;
+        R.rfbuf[((size_t)slot * R.R + ring) * 2 + 1] = ts->Vrf0;
+      }
       if (phase == 2 && (rev + 1) % P.monitor_every == 0) {
-        double *a = R.amp + ((size_t)slot * P.Nb + b) * 6;
+        double *a = R.amp + ((size_t)slot * R.Gtot + b) * 6;
         for (int pl = 0; pl < 3; pl++) {
           a[pl] = track[pl] ? mom[12 + pl] / Np : 0.0;
           a[3 + pl] = track[pl] ? P.gamma1[pl] * (st[pl] * st[pl]) + 2.0 * P.alpha1[pl] * st[pl] * st[6 + pl] +
@@ -651,16 +706,17 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
         }
       }
       if (P.rw && phase != 3) {
+        double *hist = R.rwhist + P.rw_off;
         long long n = (long long)(P.Nmlt + 2) * P.h;
         long long k = (phase == 1 ? 0 : -rev * P.h) + kb;
         long long idx = ((k % n) + n) % n;
-        R.rwhist[2 * idx] = st[1];
-        R.rwhist[2 * idx + 1] = st[2];
+        hist[2 * idx] = st[1];
+        hist[2 * idx + 1] = st[2];
         if (phase == 1)
           for (int jj = 1; jj < P.Nmlt; jj++) {
             long long i2 = ((long long)jj * P.h + kb) % n;
-            R.rwhist[2 * i2] = st[1];
-            R.rwhist[2 * i2 + 1] = st[2];
+            hist[2 * i2] = st[1];
+            hist[2 * i2 + 1] = st[2];
           }
       }
     }
@@ -685,34 +741,30 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
   }
   // ---- histograms: clamped (selffield) and dropped (LR resonators)
   const bool bad = H.bad[b] != 0;
+  const size_t lrrow = (size_t)(P.bucket_off + kb) * Nc;
   for (int i = tid; i < Nc; i += WG) {
     size_t g = (size_t)b * Nc + i;
     double cnt = (double)H.cnt[g];
     double sx = bad ? NAN : fx_to_double(H.sx[2 * g], H.sx[2 * g + 1]);
     double sz = bad ? NAN : fx_to_double(H.sz[2 * g], H.sz[2 * g + 1]);
-    if (lr_L) R.fnp_ring[(size_t)kb * Nc + i] = cnt;
-    if (lr_H) R.fnp_ring[(size_t)P.h * Nc + (size_t)kb * Nc + i] = sx;
-    if (lr_V) R.fnp_ring[2 * (size_t)P.h * Nc + (size_t)kb * Nc + i] = sz;
+    if (lr_planes[0]) R.fnp_ring[lrrow + i] = cnt;
+    if (lr_planes[1]) R.fnp_ring[R.plane_stride + lrrow + i] = sx;
+    if (lr_planes[2]) R.fnp_ring[2 * R.plane_stride + lrrow + i] = sz;
     // clamped: out-of-range particles join the edge bins
     unsigned long long xl = H.sx[2 * g], xh = H.sx[2 * g + 1], zl = H.sz[2 * g], zh = H.sz[2 * g + 1];
     unsigned int c = H.cnt[g];
-    if (i == 0 || i == Nc - 1) {
-      int t = (i == 0) ? 0 : 1;
-      // bin 0 takes the low trash, bin Nc-1 the high trash (both if Nc == 1)
-      for (int tt = 0; tt < 2; tt++) {
-        if (!((tt == 0 && i == 0) || (tt == 1 && i == Nc - 1))) continue;
-        c += H.tcnt[2 * b + tt];
-        unsigned long long al = H.tsx[4 * b + 2 * tt], ah = H.tsx[4 * b + 2 * tt + 1];
-        unsigned long long nl = xl + al;
-        xh = xh + ah + (nl < xl ? 1ULL : 0ULL);
-        xl = nl;
-        al = H.tsz[4 * b + 2 * tt];
-        ah = H.tsz[4 * b + 2 * tt + 1];
-        nl = zl + al;
-        zh = zh + ah + (nl < zl ? 1ULL : 0ULL);
-        zl = nl;
-      }
-      (void)t;
+    for (int tt = 0; tt < 2; tt++) {
+      if (!((tt == 0 && i == 0) || (tt == 1 && i == Nc - 1))) continue;
+      c += H.tcnt[2 * b + tt];
+      unsigned long long al = H.tsx[4 * b + 2 * tt], ah = H.tsx[4 * b + 2 * tt + 1];
+      unsigned long long nl = xl + al;
+      xh = xh + ah + (nl < xl ? 1ULL : 0ULL);
+      xl = nl;
+      al = H.tsz[4 * b + 2 * tt];
+      ah = H.tsz[4 * b + 2 * tt + 1];
+      nl = zl + al;
+      zh = zh + ah + (nl < zl ? 1ULL : 0ULL);
+      zl = nl;
     }
     fnp[i] = (double)c;
     dV[i] = bad ? NAN : fx_to_double(zl, zh);
@@ -722,8 +774,6 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
     R.dipH_out[g] = dH[i];
   }
   if (tid == 0) {
-    icmin_s = Nc - 1;
-    icmax_s = 0;
     // trash counters: the previous pending binning is consumed by this turn's kick
     if (phase == 2) {
       R.trash_tot[2 * b] += R.trash_pend[2 * b];
@@ -769,11 +819,12 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
   if (phase == 2 && P.scan == 1 && (rev + 1) % P.NrevScan == 0) Ib += P.scan_step / (((double)P.Nb) * 1.0e3);
   const double factG = -P.T0 * Ib / (R.np[b] * P.E0 * 1.0e9);
   const double sgm = P.sigma_tau, sgm2 = sgm * sgm, dT2 = P.dT * P.dT;
+  const double *Gl = R.G + (size_t)ring * 3 * Nc, *Gv = Gl + Nc, *Gh = Gv + Nc;
   for (int i = tid; i < Nc; i += WG) {
     double GL1 = 0.0, GV = 0.0, GH = 0.0;
     if (P.convL && i >= icmin) {
       double Gn = 0.0;
-      for (int k = i; k >= icmin; k--) Gn += fnp[k] * R.Gl[i - k];
+      for (int k = i; k >= icmin; k--) Gn += fnp[k] * Gl[i - k];
       GL1 = 1 * Gn;
       // resistive and inductive smoothing, src/transform_weak.c:670-683; reads
       // outside [0, Ncell) count as zero
@@ -791,12 +842,12 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
     }
     if (P.kickV) {
       double Gn = 0.0;
-      for (int k = i; k >= 0; k--) Gn += dV[k] * R.Gv[i - k];
+      for (int k = i; k >= 0; k--) Gn += dV[k] * Gv[i - k];
       GV = 1 * Gn;
     }
     if (P.kickH && i >= 1) {
       double Gn = 0.0;
-      for (int k = i; k >= 0; k--) Gn += dH[k] * R.Gh[i - k];
+      for (int k = i; k >= 0; k--) Gn += dH[k] * Gh[i - k];
       GH = 1 * Gn;
     }
     R.sfL[(size_t)b * Nc + i] = factG * GL1;
@@ -807,13 +858,21 @@ __global__ __launch_bounds__(WG) void k_reduce(DevParams P, const double *partia
 
 // ------------------------------------------------------------------- k_rw
 
-__global__ __launch_bounds__(WG) void k_rw(DevParams P, const double *rwhist, const double *sqrt_tt, const double *Ibunch,
-                                           const int *nfFill, const int *bucket, double *pending, const TurnState *ts)
+// Resistive-wall long-range kick of bunch b (src/transform_weak.c:780-833): the
+// sum over (Nmlt+1) h past buckets of the ring's dipole history, stored as the
+// pending kick the next pass applies.
+__device__ void rw_sum(const DevParams &P, int b, const TurnState *ts, const double *rwhist, const double *sqrt_tt,
+                       const double *Ibunch, const int *nfFill, const int *bucket, double *pending,
+                       double (*red)[2])
 {
-  __shared__ double red[WG / 64][2];
-  const int b = blockIdx.x, tid = threadIdx.x, kb = bucket[b];
+  const int tid = threadIdx.x;
+  const int kb = bucket[b];
   const long long rev = ts->rev;
   const int h = P.h;
+  const double *hist = rwhist + P.rw_off;
+  const double *stt = sqrt_tt + P.st_off;
+  const double *Ib = Ibunch + P.bucket_off;
+  const int *fill = nfFill + P.bucket_off;
   const long long n = (long long)(P.Nmlt + 2) * h;
   const long long M = (long long)(P.Nmlt + 1) * h;
   // index of tap m in the cyclic history: (-rev h + kb + m) mod n, bucket (kb + m) mod h,
@@ -826,11 +885,11 @@ __global__ __launch_bounds__(WG) void k_rw(DevParams P, const double *rwhist, co
   const int step_h = WG % h;
   double ax = 0.0, az = 0.0;
   for (long long m = m0; m <= M; m += WG) {
-    if (nfFill[in_mod]) {
-      const double I = Ibunch[in_mod];
-      const double st = sqrt_tt[m];
-      ax += P.fNp * rwhist[2 * idx] * I / st;
-      az += P.fNp * rwhist[2 * idx + 1] * I / st;
+    if (fill[in_mod]) {
+      const double I = Ib[in_mod];
+      const double st = stt[m];
+      ax += P.fNp * hist[2 * idx] * I / st;
+      az += P.fNp * hist[2 * idx + 1] * I / st;
     }
     idx += step_n;
     if (idx >= n) idx -= n;
@@ -854,45 +913,119 @@ __global__ __launch_bounds__(WG) void k_rw(DevParams P, const double *rwhist, co
     pending[2 * b] = P.track_h ? sx * P.rwconst[0][trig] : 0.0;
     pending[2 * b + 1] = P.track_v ? sz * P.rwconst[1][trig] : 0.0;
   }
+  __syncthreads();
 }
 
-// Turn end: RW trigger counter, current scan (src/tracking.c:43-59), turn counter.
-__global__ void k_end(DevParams P, TurnState *ts, double *bunch_Ib, double *Ibunch, const int *nfFill, int scan,
-                      long long NrevScan, double scan_step)
+// Turn end of one ring: RW trigger counter, current scan (src/tracking.c:43-59),
+// turn counter.
+__device__ void end_turn(const DevParams &P, TurnState *ts, double *bunch_Ib, double *Ibunch, const int *nfFill)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   long long rev = ts->rev;
   if (P.rw) ts->m++;
-  if ((rev + 1) % NrevScan == 0) {
-    if (scan == 1) {
-      ts->Iring += scan_step;
+  if ((rev + 1) % P.NrevScan == 0) {
+    if (P.scan == 1) {
+      ts->Iring += P.scan_step;
       ts->scan_val = ts->Iring;
-      for (int k = 0; k < P.Nb; k++) bunch_Ib[k] += scan_step / (((double)P.Nb) * 1.0e3);
+      for (int k = 0; k < P.Nb; k++) bunch_Ib[P.bunch_off + k] += P.scan_step / (((double)P.Nb) * 1.0e3);
       for (int i = 0; i < P.h; i++)
-        if (nfFill[i]) Ibunch[i] += scan_step / ((double)P.Nb);
+        if (nfFill[P.bucket_off + i]) Ibunch[P.bucket_off + i] += P.scan_step / ((double)P.Nb);
     }
     ts->m = 0;
   }
   ts->rev = rev + 1;
 }
 
+// RF feedback of one ring: rffb_calc_mean_voltage_phase (src/feedback_rf.c:21-34)
+// and the solve of src/worker_weakweak.c:268-278; the phase and voltage before
+// the solve are kept for the potentials output.
+__device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_hist, int init)
+{
+  ts->phai0_prev = ts->phai0;
+  ts->Vrf0_prev = ts->Vrf0;
+  if (!ts->has_fb) return;
+  const double *fbh = fb_hist + P.fbh_off;
+  const int fb_len = P.fb_len;
+  double vs = 0, ps = 0;
+  for (int m = 0; m < fb_len; m++) {
+    vs += fbh[2 * m];
+    ps += fbh[2 * m + 1];
+  }
+  double vmbar = vs / fb_len, phibar = ps / fb_len;
+  vmbar = vmbar * P.E0 * 1.0e3;
+  double num = P.vrf_design * sin(P.phi0_design) - vmbar * sin(phibar);
+  double ph = atan2(num, P.vrf_design * cos(P.phi0_design) - vmbar * cos(phibar));
+  ts->phai0 = ph;
+  ts->Vrf0 = num / sin(ph);
+  if (init && P.fb_switch_off) ts->has_fb = 0;
+}
+
+// Per bunch: the resistive-wall sum; the last block of each ring then ends the
+// ring's turn (and, with fb_here, runs its RF-feedback step).
+__global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, const int *bunch_ring, const double *rwhist,
+                                           const double *sqrt_tt, double *Ibunch, const int *nfFill,
+                                           const int *bucket, double *pending, TurnState *tsr,
+                                           unsigned *ring_done, double *bunch_Ib, int fb_here)
+{
+  __shared__ double red[WG / 64][2];
+  __shared__ int last;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int ring = bunch_ring[b];
+  const DevParams &P = Pr[ring];
+  TurnState *ts = tsr + ring;
+  if (P.rw) rw_sum(P, b, ts, rwhist, sqrt_tt, Ibunch, nfFill, bucket, pending, red);
+  // the ring's last block to finish ends the turn: every block of the ring has
+  // read the turn state and the bucket currents before its increment
+  if (tid == 0) {
+    __threadfence();
+    last = atomicAdd(&ring_done[ring], 1u) == (unsigned)(P.Nb - 1);
+  }
+  __syncthreads();
+  if (last && tid == 0) {
+    __threadfence();
+    ring_done[ring] = 0;
+    end_turn(P, ts, bunch_Ib, Ibunch, nfFill);
+    if (fb_here) fb_solve(P, ts, nullptr, 0);
+  }
+}
+
+// Turn end, one block per ring, for turns without a resistive-wall pass;
+// with fb_here also the RF-feedback step (no long-range resonators anywhere).
+__global__ void k_end(const DevParams *__restrict__ Pr, TurnState *tsr, double *bunch_Ib, double *Ibunch,
+                      const int *nfFill, int fb_here)
+{
+  if (threadIdx.x != 0) return;
+  const int ring = blockIdx.x;
+  TurnState *ts = tsr + ring;
+  end_turn(Pr[ring], ts, bunch_Ib, Ibunch, nfFill);
+  if (fb_here) fb_solve(Pr[ring], ts, nullptr, 0);
+}
+
 // ------------------------------------------------------------------- k_lr
 
-// One lane per long-range resonator: the recurrence of construct_wake_phasor
-// (src/transform_weak.c:961-1048) or, with init_turns > 0, wake_phasor_init
-// (:869-952).  Constants come from the host (glibc exp/cos/sin), the update
-// order is the reference's, so the phasors are bit-identical.
-__global__ void k_lr(int nres, const LRConst *lc, const double *fnp_ring, const double *Ibunch, const int *nfFill,
-                     int h, int Nc, double *phasor_end, double *lrc, double *fb_hist, int fb_len, TurnState *ts,
-                     int init_turns)
+// One lane per long-range resonator (of any ring): the recurrence of
+// construct_wake_phasor (src/transform_weak.c:961-1048) or, with init set,
+// wake_phasor_init (:869-952).  Constants come from the host (glibc exp/cos/sin),
+// the update order is the reference's, so the phasors are bit-identical.
+__global__ void k_lr(int nres, const LRConst *lc, const DevParams *__restrict__ Pr, const double *fnp_ring,
+                     long long plane_stride, const double *Ibunch, const int *nfFill, int Nc, double *phasor_end,
+                     double *lrc, double *fb_hist, TurnState *tsr, int init)
 {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nres) return;
   const LRConst L = lc[r];
-  const double *fnp = fnp_ring + (size_t)L.plane * h * Nc;
-  double *out = lrc + (size_t)r * h * Nc;
+  const DevParams &P = Pr[L.ring];
+  TurnState *ts = tsr + L.ring;
+  const int h = L.h;
+  const double *fnp = fnp_ring + (size_t)L.plane * plane_stride + (size_t)L.bucket_off * Nc;
+  const double *Ib = Ibunch + L.bucket_off;
+  const int *fill = nfFill + L.bucket_off;
+  double *out = lrc + L.lrc_off;
+  double *fbh = fb_hist + P.fbh_off;
+  const int fb_len = P.fb_len;
+  const int nturn = init ? L.init_turns : 1;
+  if (nturn <= 0) return;
   double Vo0, Vo1, Vn0, Vn1;
-  if (init_turns) {
+  if (init) {
     Vo0 = 0.0;
     Vo1 = 0.0;
   } else {
@@ -903,25 +1036,24 @@ __global__ void k_lr(int nres, const LRConst *lc, const double *fnp_ring, const 
   Vn1 = phasor_end[2 * r + 1];
   const bool fb = L.fb && ts->has_fb;
   long long cnt = ts->fb_count;
-  const int nturn = init_turns ? init_turns : 1;
   for (int k = 0; k < nturn; k++)
     for (int m = 0; m < h; m++) {
       int i = h - m - 1;
       if (fb) {
         double vr = Vn0 * L.pb0 - Vn1 * L.pb1, vi = Vn0 * L.pb1 + Vn1 * L.pb0;
         unsigned key = ((unsigned)cnt % (unsigned)fb_len) * 2;
-        fb_hist[key] = sqrt(vr * vr + vi * vi);
-        fb_hist[key + 1] = atan2(vr, vi);
+        fbh[key] = sqrt(vr * vr + vi * vi);
+        fbh[key + 1] = atan2(vr, vi);
         cnt++;
       }
-      if (nfFill[i] == 1) {
-        double Ii = Ibunch[i];
+      if (fill[i] == 1) {
+        double Ii = Ib[i];
         const double *f = fnp + (size_t)i * Nc;
         double *o = out + (size_t)i * Nc;
         for (int j = 0; j < Nc; j++) {
           Vn0 = (Vo0 * L.pr0 - Vo1 * L.pr1) - Ii * f[j] * L.fac;
           Vn1 = (Vo0 * L.pr1 + Vo1 * L.pr0);
-          if (!init_turns) o[j] = Vn0 + 0.5 * Ii * f[j] * L.fac;
+          if (!init) o[j] = Vn0 + 0.5 * Ii * f[j] * L.fac;
           Vo0 = Vn0;
           Vo1 = Vn1;
         }
@@ -938,8 +1070,8 @@ __global__ void k_lr(int nres, const LRConst *lc, const double *fnp_ring, const 
       if (fb) {
         double vr = Vn0 * L.pa0 - Vn1 * L.pa1, vi = Vn0 * L.pa1 + Vn1 * L.pa0;
         unsigned key = ((unsigned)cnt % (unsigned)fb_len) * 2;
-        fb_hist[key] = sqrt(vr * vr + vi * vi);
-        fb_hist[key + 1] = atan2(vr, vi);
+        fbh[key] = sqrt(vr * vr + vi * vi);
+        fbh[key + 1] = atan2(vr, vi);
         cnt++;
       }
     }
@@ -948,91 +1080,94 @@ __global__ void k_lr(int nres, const LRConst *lc, const double *fnp_ring, const 
   if (fb) ts->fb_count = cnt;
 }
 
-// lr_wake[plane][i][j] = sum over the plane's resonators in order of their terms
-__global__ void k_lr_sum(int nres, const LRConst *lc, const double *lrc, const int *nfFill, int h, int Nc, double *lr_wake)
+// lr_wake[plane][global bucket][j] = sum over the ring's resonators of that
+// plane, in order, of their terms
+__global__ void k_lr_sum(const int *lr_first, const LRConst *lc, const double *lrc, const int *nfFill,
+                         const int *bucket_ring, int H, int Nc, double *lr_wake)
 {
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long per = (long long)h * Nc;
+  long long per = (long long)H * Nc;
   if (e >= 3 * per) return;
   int plane = (int)(e / per);
-  long long ij = e % per;
-  int i = (int)(ij / Nc);
-  if (!nfFill[i]) {
+  long long qj = e % per;
+  int q = (int)(qj / Nc);
+  int j = (int)(qj % Nc);
+  if (!nfFill[q]) {
     lr_wake[e] = 0.0;
     return;
   }
+  const int ring = bucket_ring[q];
   double s = 0.0;
-  for (int r = 0; r < nres; r++)
-    if (lc[r].plane == plane) s += lrc[(size_t)r * per + ij];
+  // the ring's resonators are contiguous in lc (ring-major, then plane, then index)
+  for (int r = lr_first[ring]; r < lr_first[ring + 1]; r++)
+    if (lc[r].plane == plane) s += lrc[lc[r].lrc_off + (size_t)(q - lc[r].bucket_off) * Nc + j];
   lr_wake[e] = s;
 }
 
-// RF feedback: rffb_calc_mean_voltage_phase (src/feedback_rf.c:21-34) and the
-// solve of src/worker_weakweak.c:268-278.
-__global__ void k_fb_solve(TurnState *ts, const double *fb_hist, int fb_len, double E0, double vrf_design,
-                           double phi0_design, int switch_off)
+// RF feedback, one lane per ring: rffb_calc_mean_voltage_phase
+// (src/feedback_rf.c:21-34) and the solve of src/worker_weakweak.c:268-278.
+__global__ void k_fb_solve(int nrings, const DevParams *__restrict__ Pr, TurnState *tsr, const double *fb_hist,
+                           int init)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  ts->phai0_prev = ts->phai0;
-  ts->Vrf0_prev = ts->Vrf0;
-  if (!ts->has_fb) return;
-  double vs = 0, ps = 0;
-  for (int m = 0; m < fb_len; m++) {
-    vs += fb_hist[2 * m];
-    ps += fb_hist[2 * m + 1];
-  }
-  double vmbar = vs / fb_len, phibar = ps / fb_len;
-  vmbar = vmbar * E0 * 1.0e3;
-  double num = vrf_design * sin(phi0_design) - vmbar * sin(phibar);
-  double ph = atan2(num, vrf_design * cos(phi0_design) - vmbar * cos(phibar));
-  ts->phai0 = ph;
-  ts->Vrf0 = num / sin(ph);
-  if (switch_off) ts->has_fb = 0;
-}
-
-__global__ void k_record_rf(const TurnState *ts, double *rfbuf)
-{
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int slot = (int)(ts->rev % STATS_TURNS);
-    rfbuf[2 * slot] = ts->phai0;
-    rfbuf[2 * slot + 1] = ts->Vrf0;
-  }
+  int ring = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ring >= nrings) return;
+  fb_solve(Pr[ring], tsr + ring, fb_hist, init);
 }
 
 // =============================================================== host side
 
+struct RingHost {
+  mbt_config cfg; // arrays below are owned copies
+  int *nfFill, *bucket, *bunch_out;
+  double *Ibunch, *bunch_Ib, *Ib_frac;
+  int planes[3];
+  int lr_plane[3];
+};
+
+#define NGRAPH 4
+struct GraphEntry {
+  hipGraphExec_t exec;
+  long long phase; // first turn modulo NrevMon
+  long n;          // turns
+  int timed;       // records the timing events
+};
+
 struct mbt_ring {
-  mbt_config cfg;
-  int *nfFill_h, *bucket_h, *bunch_out_h;
-  double *Ibunch_h, *bunch_Ib_h, *Ib_frac_h;
+  int R;
+  RingHost *rh;
+  DevParams *Ph; // host copies of the per-ring parameters
   int device;
   hipStream_t stream;
-  DevParams P;
-  int Nb, Nc, h;
+  int G, H, Nc;    // bunches, buckets (all rings), slices
   long long Ntot;
   int nwg;
+  int any_qe, any_dip, any_gen, nrev_mon;
+  int sf_any;
   Coords C;
-  long long *wg_first, *wg_end;
-  int *wg_bunch, *bunch_wg0, *bucket, *np, *nfFill;
+  long long *wg_first, *wg_end, *bunch_first;
+  int *wg_bunch, *bunch_wg0, *bucket, *bunch_ring, *bucket_ring, *np, *nfFill, *lr_planes;
+  DevParams *P;
   double *partial, *shift;
-  HistPtr H;
-  RedOut R;
+  HistPtr Hs;
+  RedOut Ro;
   double *sfL, *sfV, *sfH, *lr_wake, *fnp_ring, *lrc;
-  double *Gl, *Gv, *Gh;
+  double *Gf;
   double *pending, *rwhist, *sqrt_tt, *Ibunch, *bunch_Ib;
   double *stats, *amp, *rfbuf;
   double *fnp_out, *dipV_out, *dipH_out;
   int *trash_tot, *trash_pend;
   TurnState *ts;
+  unsigned *ring_done; // [R] blocks of the ring done with the turn's RW pass
+  GraphEntry graphs[NGRAPH];
+  int graph_next, use_graphs;
   LRConst *lc;
+  int *lr_first; // [R+1] first resonator of each ring in lc
   int nres;
   double *phasor_end, *fb_hist;
-  int fb_len, fb_off;
-  int lr_plane[3];
+  long long rw_total, st_total, fbh_total;
   size_t lds_turn, lds_red;
   long long rev_host;
-  hipEvent_t ev[8];
-  hipEvent_t *evpool; // 5 per turn of a stats chunk when timing
+  hipEvent_t *evpool; // 6 per turn of a stats chunk when timing
   double kt_ms[4];
   long kt_launches[4];
   int timing;
@@ -1043,10 +1178,8 @@ static size_t turn_lds(int Nc)
   return (size_t)(4 * Nc + 8) * 8 + (size_t)3 * Nc * 8 + (size_t)(Nc + 2) * 4 + 16;
 }
 
-static void fill_params(mbt_ring *r)
+static void fill_params(const mbt_config *c, const int planes[3], DevParams &P)
 {
-  const mbt_config *c = &r->cfg;
-  DevParams &P = r->P;
   memset(&P, 0, sizeof P);
   P.Nb = c->Nbunch;
   P.h = (int)c->Nharm;
@@ -1060,6 +1193,9 @@ static void fill_params(mbt_ring *r)
   P.rw = c->EnableRW_long != 0;
   P.n_ahc = c->n_ahc;
   P.monitor_every = (int)c->NrevMon;
+  P.scan = c->scan;
+  P.NrevScan = c->NrevScan;
+  P.scan_step = c->scan_step;
   P.seed = (unsigned)c->iseed[0];
   for (int a = 0; a < c->n_ahc; a++) {
     P.ahc_amp[a] = c->ahc[a].Vpeak / (c->E0 * MBT_FGIGA);
@@ -1071,10 +1207,6 @@ static void fill_params(mbt_ring *r)
   P.E0k = c->E0 * MBT_FKILO;
   P.RsisL = c->RsisL;
   P.aindL = c->aindL;
-  P.Np = c->Np;
-  P.scan = c->scan;
-  P.NrevScan = c->NrevScan;
-  P.scan_step = c->scan_step;
   P.wrf = c->wrf;
   P.Urad = c->Urad;
   P.damp = 1.0 - c->De;
@@ -1120,6 +1252,16 @@ static void fill_params(mbt_ring *r)
       P.rwconst[p][t] = c->T0 / c->E0 / c->fNp / MBT_FTERA * c->Lc / (M_PI * pow(beff3, 3)) *
                         sqrt(MBT_Z0 * MBT_C_LIGHT * c->rhorw / M_PI);
     }
+  int lr_tot = c->n_lr[0] + c->n_lr[1] + c->n_lr[2];
+  P.sf_active = (planes[0] + planes[1] + planes[2] + lr_tot) > 0 && c->Ncell > 0;
+  P.kickL = (c->n_lr[0] + planes[0]) > 0;
+  P.convL = planes[0] > 0;
+  P.kickV = planes[2] > 0;
+  P.kickH = planes[1] > 0;
+  P.fb_len = c->has_rf_feedback ? abs(c->rffb_len_average) : 1;
+  P.fb_switch_off = c->has_rf_feedback && c->rffb_len_average < 0;
+  P.vrf_design = c->rffb_vrf_design;
+  P.phi0_design = c->rffb_phi0_design;
 }
 
 template <typename T> static int dalloc(T **p, size_t n)
@@ -1142,61 +1284,70 @@ template <typename T> static int dalloc(T **p, size_t n)
     }                                                                                            \
   } while (0)
 
-static void copy_cfg(mbt_ring *r, const mbt_config *c)
+// uploads are ordered on the ring's stream (the host buffers live until the
+// stream is synchronised below)
+#define UP(dst, src, n) CK(hipMemcpyAsync((dst), (src), (size_t)(n) * sizeof(*(src)), hipMemcpyHostToDevice, r->stream))
+
+static void copy_cfg(RingHost *h, const mbt_config *c)
 {
-  r->cfg = *c;
+  h->cfg = *c;
   int H = (int)c->Nharm, Nb = c->Nbunch;
-  r->nfFill_h = (int *)malloc(H * sizeof(int));
-  memcpy(r->nfFill_h, c->nfFill, H * sizeof(int));
-  r->Ibunch_h = (double *)malloc(H * sizeof(double));
-  memcpy(r->Ibunch_h, c->Ibunch, H * sizeof(double));
-  r->bucket_h = (int *)malloc(Nb * sizeof(int));
-  memcpy(r->bucket_h, c->bucket, Nb * sizeof(int));
-  r->bunch_Ib_h = (double *)malloc(Nb * sizeof(double));
-  memcpy(r->bunch_Ib_h, c->bunch_Ib, Nb * sizeof(double));
-  r->Ib_frac_h = (double *)malloc(Nb * sizeof(double));
-  memcpy(r->Ib_frac_h, c->Ib_frac, Nb * sizeof(double));
-  r->bunch_out_h = (int *)malloc((c->Nbunch_out > 0 ? c->Nbunch_out : 1) * sizeof(int));
-  if (c->Nbunch_out > 0) memcpy(r->bunch_out_h, c->bunch_out, c->Nbunch_out * sizeof(int));
-  r->cfg.nfFill = r->nfFill_h;
-  r->cfg.Ibunch = r->Ibunch_h;
-  r->cfg.bucket = r->bucket_h;
-  r->cfg.bunch_Ib = r->bunch_Ib_h;
-  r->cfg.Ib_frac = r->Ib_frac_h;
-  r->cfg.bunch_out = r->bunch_out_h;
+  h->nfFill = (int *)malloc(H * sizeof(int));
+  memcpy(h->nfFill, c->nfFill, H * sizeof(int));
+  h->Ibunch = (double *)malloc(H * sizeof(double));
+  memcpy(h->Ibunch, c->Ibunch, H * sizeof(double));
+  h->bucket = (int *)malloc(Nb * sizeof(int));
+  memcpy(h->bucket, c->bucket, Nb * sizeof(int));
+  h->bunch_Ib = (double *)malloc(Nb * sizeof(double));
+  memcpy(h->bunch_Ib, c->bunch_Ib, Nb * sizeof(double));
+  h->Ib_frac = (double *)malloc(Nb * sizeof(double));
+  memcpy(h->Ib_frac, c->Ib_frac, Nb * sizeof(double));
+  h->bunch_out = (int *)malloc((c->Nbunch_out > 0 ? c->Nbunch_out : 1) * sizeof(int));
+  if (c->Nbunch_out > 0) memcpy(h->bunch_out, c->bunch_out, c->Nbunch_out * sizeof(int));
+  h->cfg.nfFill = h->nfFill;
+  h->cfg.Ibunch = h->Ibunch;
+  h->cfg.bucket = h->bucket;
+  h->cfg.bunch_Ib = h->bunch_Ib;
+  h->cfg.Ib_frac = h->Ib_frac;
+  h->cfg.bunch_out = h->bunch_out;
 }
 
 extern "C" void mbt_ring_destroy(mbt_ring *r)
 {
   if (!r) return;
-  hipSetDevice(r->device);
-  void *ptrs[] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp, r->wg_first, r->wg_end, r->wg_bunch,
-                  r->bunch_wg0, r->bucket, r->np, r->nfFill, r->partial, r->shift, r->H.cnt, r->H.sx, r->H.sz,
-                  r->H.tcnt, r->H.tsx, r->H.tsz, r->H.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc,
-                  r->Gl, r->Gv, r->Gh, r->pending, r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp,
-                  r->rfbuf, r->fnp_out, r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->lc,
-                  r->phasor_end, r->fb_hist};
+  (void)hipSetDevice(r->device);
+  void *ptrs[] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp, r->wg_first, r->wg_end, r->bunch_first,
+                  r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
+                  r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
+                  r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
+                  r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
+                  r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
-    if (ptrs[i]) hipFree(ptrs[i]);
-  for (int i = 0; i < 8; i++)
-    if (r->ev[i]) hipEventDestroy(r->ev[i]);
+    if (ptrs[i]) (void)hipFree(ptrs[i]);
   if (r->evpool) {
-    for (int i = 0; i < 6 * STATS_TURNS; i++) hipEventDestroy(r->evpool[i]);
+    for (int i = 0; i < 6 * STATS_TURNS; i++) (void)hipEventDestroy(r->evpool[i]);
     free(r->evpool);
   }
-  if (r->stream) hipStreamDestroy(r->stream);
-  free(r->nfFill_h);
-  free(r->Ibunch_h);
-  free(r->bucket_h);
-  free(r->bunch_Ib_h);
-  free(r->Ib_frac_h);
-  free(r->bunch_out_h);
+  for (int i = 0; i < NGRAPH; i++)
+    if (r->graphs[i].exec) (void)hipGraphExecDestroy(r->graphs[i].exec);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  if (r->rh)
+    for (int k = 0; k < r->R; k++) {
+      RingHost *h = &r->rh[k];
+      free(h->nfFill);
+      free(h->Ibunch);
+      free(h->bucket);
+      free(h->bunch_Ib);
+      free(h->Ib_frac);
+      free(h->bunch_out);
+    }
+  free(r->rh);
+  free(r->Ph);
   free(r);
 }
 
-extern "C" int mbt_ring_create(const mbt_config *c, int device, int flags, mbt_ring **out)
+static int check_cfg(const mbt_config *c)
 {
-  *out = NULL;
   if (c->bunch_model != 0) {
     mbt_set_error("BunchModel STRONG is handled by the weak-strong driver, not the macroparticle ring");
     return MBT_EUNSUPPORTED;
@@ -1207,6 +1358,11 @@ extern "C" int mbt_ring_create(const mbt_config *c, int device, int flags, mbt_r
   }
   if (c->NrevMon <= 0) {
     mbt_set_error("NrevMon must be positive");
+    return MBT_EINVAL;
+  }
+  int lr_tot = c->n_lr[0] + c->n_lr[1] + c->n_lr[2];
+  if ((lr_tot > 0 || c->n_res > 0 || c->n_wakefile > 0) && c->Ncell <= 0) {
+    mbt_set_error("wakes need Ncell > 0");
     return MBT_EINVAL;
   }
   if (c->has_rf_feedback && c->n_lr[0] == 0) {
@@ -1221,211 +1377,325 @@ extern "C" int mbt_ring_create(const mbt_config *c, int device, int flags, mbt_r
     mbt_set_error("averaging_length must be nonzero");
     return MBT_EINVAL;
   }
+  return MBT_OK;
+}
+
+static int create_rings(const mbt_config *const *cfgs, int R, int device, int flags, mbt_ring **out)
+{
+  *out = NULL;
+  if (R <= 0) {
+    mbt_set_error("no ring");
+    return MBT_EINVAL;
+  }
+  for (int k = 0; k < R; k++) {
+    int rc = check_cfg(cfgs[k]);
+    if (rc) return rc;
+    if (cfgs[k]->Ncell != cfgs[0]->Ncell || cfgs[k]->NrevMon != cfgs[0]->NrevMon) {
+      mbt_set_error("rings of one ensemble share Ncell and NrevMon");
+      return MBT_EINVAL;
+    }
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
     mbt_set_error("no HIP device %d", device);
     return MBT_EDEVICE;
   }
   mbt_ring *r = (mbt_ring *)calloc(1, sizeof(mbt_ring));
-  copy_cfg(r, c);
+  r->R = R;
   r->device = device;
+  r->rh = (RingHost *)calloc(R, sizeof(RingHost));
+  r->Ph = (DevParams *)calloc(R, sizeof(DevParams));
+  for (int k = 0; k < R; k++) copy_cfg(&r->rh[k], cfgs[k]);
   CK(hipSetDevice(device));
-  CK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-  for (int i = 0; i < 8; i++) CK(hipEventCreate(&r->ev[i]));
-  fill_params(r);
-  const mbt_config *cf = &r->cfg;
-  const int Nb = cf->Nbunch, Nc = cf->Ncell, h = (int)cf->Nharm;
-  r->Nb = Nb;
+  // a blocking stream: ordered after work on the null stream (allocation-time
+  // memsets) like every copy the library makes
+  CK(hipStreamCreate(&r->stream));
+  const int Nc = cfgs[0]->Ncell, NcA = Nc > 0 ? Nc : 1;
   r->Nc = Nc;
-  r->h = h;
-  const long long Np = cf->Np;
-  r->Ntot = Np * Nb;
-  // ---- green functions
-  double *Gl = (double *)calloc(Nc > 0 ? Nc : 1, sizeof(double));
-  double *Gv = (double *)calloc(Nc > 0 ? Nc : 1, sizeof(double));
-  double *Gh = (double *)calloc(Nc > 0 ? Nc : 1, sizeof(double));
-  int planes[3];
-  int rc = mbt_greens_functions(cf, Gl, Gv, Gh, planes);
-  if (rc) {
-    free(Gl);
-    free(Gv);
-    free(Gh);
-    mbt_ring_destroy(r);
-    return rc;
+  r->nrev_mon = (int)cfgs[0]->NrevMon;
+  // ---- per-ring parameters, Green functions, offsets
+  double *Gf = (double *)calloc((size_t)R * 3 * NcA, sizeof(double));
+  int G = 0, H = 0;
+  long long Ntot = 0, rw_total = 0, st_total = 0, fbh_total = 0;
+  for (int k = 0; k < R; k++) {
+    RingHost *hh = &r->rh[k];
+    const mbt_config *c = &hh->cfg;
+    int rc = mbt_greens_functions(c, Gf + (size_t)k * 3 * NcA, Gf + (size_t)k * 3 * NcA + NcA,
+                                  Gf + (size_t)k * 3 * NcA + 2 * NcA, hh->planes);
+    if (rc) {
+      free(Gf);
+      mbt_ring_destroy(r);
+      return rc;
+    }
+    DevParams &P = r->Ph[k];
+    fill_params(c, hh->planes, P);
+    P.ring = k;
+    P.bunch_off = G;
+    P.bucket_off = H;
+    P.rw_off = rw_total;
+    P.st_off = st_total;
+    P.fbh_off = fbh_total;
+    G += c->Nbunch;
+    H += (int)c->Nharm;
+    Ntot += (long long)c->Np * c->Nbunch;
+    if (c->EnableRW_long) {
+      rw_total += 2LL * (c->Nmlt + 2) * c->Nharm;
+      st_total += (long long)(c->Nmlt + 1) * c->Nharm + 1;
+    }
+    fbh_total += 2LL * P.fb_len;
+    hh->lr_plane[0] = c->n_lr[0] > 0;
+    hh->lr_plane[1] = c->n_lr[1] > 0 && c->TrackPlane[1];
+    hh->lr_plane[2] = c->n_lr[2] > 0 && c->TrackPlane[2];
+    r->any_qe |= P.qe;
+    r->any_dip |= P.kickV || P.kickH || hh->lr_plane[1] || hh->lr_plane[2];
+    r->any_gen |= P.ideal_hc || P.n_ahc > 0 || P.amp_dq;
+    r->sf_any |= P.sf_active;
   }
-  int lr_tot = cf->n_lr[0] + cf->n_lr[1] + cf->n_lr[2];
-  r->P.sf_active = (planes[0] + planes[1] + planes[2] + lr_tot) > 0 && Nc > 0;
-  r->P.kickL = (cf->n_lr[0] + planes[0]) > 0;
-  r->P.convL = planes[0] > 0;
-  r->P.kickV = planes[2] > 0;
-  r->P.kickH = planes[1] > 0;
-  r->lr_plane[0] = cf->n_lr[0] > 0;
-  r->lr_plane[1] = cf->n_lr[1] > 0 && cf->TrackPlane[1];
-  r->lr_plane[2] = cf->n_lr[2] > 0 && cf->TrackPlane[2];
+  r->G = G;
+  r->H = H;
+  r->Ntot = Ntot;
+  r->rw_total = rw_total;
+  r->st_total = st_total;
+  r->fbh_total = fbh_total;
   // ---- workgroup decomposition: contiguous chunks of one bunch
   const long long CH = (long long)WG * CHUNK_ITEMS;
-  const long long per_b = (Np + CH - 1) / CH;
-  r->nwg = (int)(per_b * Nb);
-  long long *wf = (long long *)malloc(r->nwg * sizeof(long long)), *we = (long long *)malloc(r->nwg * sizeof(long long));
-  int *wb = (int *)malloc(r->nwg * sizeof(int)), *b0 = (int *)malloc((Nb + 1) * sizeof(int));
-  int *nph = (int *)malloc(Nb * sizeof(int));
-  for (int b = 0, w = 0; b < Nb; b++) {
-    b0[b] = w;
-    nph[b] = (int)Np;
-    for (long long k = 0; k < per_b; k++, w++) {
-      wf[w] = b * Np + k * CH;
-      we[w] = b * Np + ((k + 1) * CH < Np ? (k + 1) * CH : Np);
-      wb[w] = b;
-    }
-  }
-  b0[Nb] = r->nwg;
-  DA(r->C.tau, r->Ntot);
-  DA(r->C.del, r->Ntot);
-  DA(r->C.x, r->Ntot);
-  DA(r->C.xp, r->Ntot);
-  DA(r->C.z, r->Ntot);
-  DA(r->C.zp, r->Ntot);
-  DA(r->wg_first, r->nwg);
-  DA(r->wg_end, r->nwg);
-  DA(r->wg_bunch, r->nwg);
-  DA(r->bunch_wg0, Nb + 1);
-  DA(r->bucket, Nb);
-  DA(r->np, Nb);
-  DA(r->nfFill, h);
-  DA(r->partial, (size_t)r->nwg * NMOM);
-  DA(r->shift, (size_t)Nb * 6);
-  int NcA = Nc > 0 ? Nc : 1;
-  DA(r->H.cnt, (size_t)Nb * NcA);
-  DA(r->H.sx, (size_t)Nb * NcA * 2);
-  DA(r->H.sz, (size_t)Nb * NcA * 2);
-  DA(r->H.tcnt, (size_t)Nb * 2);
-  DA(r->H.tsx, (size_t)Nb * 4);
-  DA(r->H.tsz, (size_t)Nb * 4);
-  DA(r->H.bad, Nb);
-  DA(r->sfL, (size_t)Nb * NcA);
-  DA(r->sfV, (size_t)Nb * NcA);
-  DA(r->sfH, (size_t)Nb * NcA);
-  DA(r->lr_wake, (size_t)3 * h * NcA);
-  DA(r->fnp_ring, (size_t)3 * h * NcA);
-  DA(r->Gl, NcA);
-  DA(r->Gv, NcA);
-  DA(r->Gh, NcA);
-  DA(r->pending, (size_t)Nb * 2);
-  long long M = (long long)(cf->Nmlt + 1) * h;
-  DA(r->rwhist, (size_t)(cf->EnableRW_long ? (cf->Nmlt + 2) * h * 2 : 2));
-  DA(r->sqrt_tt, (size_t)(cf->EnableRW_long ? M + 1 : 1));
-  DA(r->Ibunch, h);
-  DA(r->bunch_Ib, Nb);
-  DA(r->stats, (size_t)STATS_TURNS * Nb * 12);
-  DA(r->amp, (size_t)STATS_TURNS * Nb * 6);
-  DA(r->rfbuf, (size_t)STATS_TURNS * 2);
-  DA(r->fnp_out, (size_t)Nb * NcA);
-  DA(r->dipV_out, (size_t)Nb * NcA);
-  DA(r->dipH_out, (size_t)Nb * NcA);
-  DA(r->trash_tot, (size_t)Nb * 2);
-  DA(r->trash_pend, (size_t)Nb * 2);
-  DA(r->ts, 1);
-  // ---- long-range resonator constants (host libm, reference formulas)
-  LRConst lc[MAXLR];
-  int nres = 0;
-  for (int p = 0; p < 3; p++) {
-    if (!r->lr_plane[p]) continue;
-    for (int l = 0; l < cf->n_lr[p]; l++) {
-      const mbt_lr_resonator *res = &cf->lr[p][l];
-      LRConst &L = lc[nres++];
-      L.plane = p;
-      L.l = l;
-      double alpha = res->wr * 0.5 / res->Qfactor;
-      double Amp = 2 * alpha * res->Rs * cf->T0 / cf->E0 / MBT_FGIGA;
-      double dTau = cf->dT * cf->sigma_tau;
-      double tbucket = (cf->T0 / cf->h - Nc * dTau);
-      L.fac = Amp / (unsigned)cf->Np * MBT_FMILLI;
-      double C0 = -alpha, C1 = res->wr;
-      L.pr0 = exp(C0 * dTau) * cos(C1 * dTau);
-      L.pr1 = exp(C0 * dTau) * sin(C1 * dTau);
-      L.ge = exp(C0 * tbucket);
-      L.gc = cos(C1 * tbucket);
-      L.gs = sin(C1 * tbucket);
-      double tb = cf->T0 / cf->h;
-      L.ee = exp(C0 * tb);
-      L.ec = cos(C1 * tb);
-      L.es = sin(C1 * tb);
-      L.fb = cf->has_rf_feedback && p == 0 && cf->rffb_resonator == l + 1;
-      double taubeam = 0, taub2beam = 0;
-      if (L.fb) {
-        taubeam = cf->Nsigma * cf->sigma_tau + dTau * 1.5;
-        taub2beam = -cf->Nsigma * cf->sigma_tau + dTau * 0.5 - tbucket;
+  int nwg = 0;
+  for (int k = 0; k < R; k++) nwg += (int)(((long long)r->rh[k].cfg.Np + CH - 1) / CH) * r->rh[k].cfg.Nbunch;
+  r->nwg = nwg;
+  long long *wf = (long long *)malloc(nwg * sizeof(long long)), *we = (long long *)malloc(nwg * sizeof(long long));
+  int *wb = (int *)malloc(nwg * sizeof(int)), *b0 = (int *)malloc((G + 1) * sizeof(int));
+  int *nph = (int *)malloc(G * sizeof(int)), *bring = (int *)malloc(G * sizeof(int)), *bk = (int *)malloc(G * sizeof(int));
+  long long *bfirst = (long long *)malloc(G * sizeof(long long));
+  int *qring = (int *)malloc(H * sizeof(int)), *fill = (int *)malloc(H * sizeof(int));
+  double *Ibq = (double *)malloc(H * sizeof(double)), *bIb = (double *)malloc(G * sizeof(double));
+  {
+    int w = 0, g = 0, q = 0;
+    long long pos = 0;
+    for (int k = 0; k < R; k++) {
+      const mbt_config *c = &r->rh[k].cfg;
+      const long long Np = c->Np, per_b = (Np + CH - 1) / CH;
+      for (int i = 0; i < (int)c->Nharm; i++, q++) {
+        qring[q] = k;
+        fill[q] = c->nfFill[i];
+        Ibq[q] = c->Ibunch[i];
       }
-      L.pb0 = exp(C0 * taubeam) * cos(C1 * taubeam);
-      L.pb1 = exp(C0 * taubeam) * sin(C1 * taubeam);
-      L.pa0 = exp(C0 * taub2beam) * cos(C1 * taub2beam);
-      L.pa1 = exp(C0 * taub2beam) * sin(C1 * taub2beam);
+      for (int bb = 0; bb < c->Nbunch; bb++, g++) {
+        b0[g] = w;
+        nph[g] = (int)Np;
+        bring[g] = k;
+        bk[g] = c->bucket[bb];
+        bIb[g] = c->bunch_Ib[bb];
+        bfirst[g] = pos;
+        for (long long kk = 0; kk < per_b; kk++, w++) {
+          wf[w] = pos + kk * CH;
+          we[w] = pos + ((kk + 1) * CH < Np ? (kk + 1) * CH : Np);
+          wb[w] = g;
+        }
+        pos += Np;
+      }
+    }
+    b0[G] = w;
+  }
+  DA(r->C.tau, Ntot);
+  DA(r->C.del, Ntot);
+  DA(r->C.x, Ntot);
+  DA(r->C.xp, Ntot);
+  DA(r->C.z, Ntot);
+  DA(r->C.zp, Ntot);
+  DA(r->wg_first, nwg);
+  DA(r->wg_end, nwg);
+  DA(r->wg_bunch, nwg);
+  DA(r->bunch_wg0, G + 1);
+  DA(r->bunch_first, G);
+  DA(r->bucket, G);
+  DA(r->bunch_ring, G);
+  DA(r->bucket_ring, H);
+  DA(r->np, G);
+  DA(r->nfFill, H);
+  DA(r->lr_planes, 3);
+  DA(r->P, R);
+  DA(r->partial, (size_t)nwg * NMOM);
+  DA(r->shift, (size_t)G * 6);
+  DA(r->Hs.cnt, (size_t)G * NcA);
+  DA(r->Hs.sx, (size_t)G * NcA * 2);
+  DA(r->Hs.sz, (size_t)G * NcA * 2);
+  DA(r->Hs.tcnt, (size_t)G * 2);
+  DA(r->Hs.tsx, (size_t)G * 4);
+  DA(r->Hs.tsz, (size_t)G * 4);
+  DA(r->Hs.bad, G);
+  DA(r->sfL, (size_t)G * NcA);
+  DA(r->sfV, (size_t)G * NcA);
+  DA(r->sfH, (size_t)G * NcA);
+  DA(r->lr_wake, (size_t)3 * H * NcA);
+  DA(r->fnp_ring, (size_t)3 * H * NcA);
+  DA(r->Gf, (size_t)R * 3 * NcA);
+  DA(r->pending, (size_t)G * 2);
+  DA(r->rwhist, rw_total > 0 ? rw_total : 2);
+  DA(r->sqrt_tt, st_total > 0 ? st_total : 1);
+  DA(r->Ibunch, H);
+  DA(r->bunch_Ib, G);
+  DA(r->stats, (size_t)STATS_TURNS * G * 12);
+  DA(r->amp, (size_t)STATS_TURNS * G * 6);
+  DA(r->rfbuf, (size_t)STATS_TURNS * R * 2);
+  DA(r->fnp_out, (size_t)G * NcA);
+  DA(r->dipV_out, (size_t)G * NcA);
+  DA(r->dipH_out, (size_t)G * NcA);
+  DA(r->trash_tot, (size_t)G * 2);
+  DA(r->trash_pend, (size_t)G * 2);
+  DA(r->ts, R);
+  DA(r->ring_done, R);
+  DA(r->fb_hist, fbh_total > 0 ? fbh_total : 2);
+  // ---- long-range resonator constants (host libm, reference formulas)
+  int nres = 0;
+  for (int k = 0; k < R; k++)
+    for (int p = 0; p < 3; p++)
+      if (r->rh[k].lr_plane[p]) nres += r->rh[k].cfg.n_lr[p];
+  LRConst *lc = (LRConst *)calloc(nres > 0 ? nres : 1, sizeof(LRConst));
+  int *lrf = (int *)calloc(R + 1, sizeof(int));
+  long long lrc_total = 0;
+  {
+    int ir = 0;
+    for (int k = 0; k < R; k++) {
+      lrf[k] = ir;
+      const mbt_config *cf = &r->rh[k].cfg;
+      for (int p = 0; p < 3; p++) {
+        if (!r->rh[k].lr_plane[p]) continue;
+        for (int l = 0; l < cf->n_lr[p]; l++) {
+          const mbt_lr_resonator *res = &cf->lr[p][l];
+          LRConst &L = lc[ir++];
+          L.ring = k;
+          L.plane = p;
+          L.l = l;
+          L.h = (int)cf->Nharm;
+          L.bucket_off = r->Ph[k].bucket_off;
+          L.lrc_off = lrc_total;
+          L.init_turns = (int)(cf->Nbumax / cf->h);
+          lrc_total += (long long)cf->Nharm * NcA;
+          double alpha = res->wr * 0.5 / res->Qfactor;
+          double Amp = 2 * alpha * res->Rs * cf->T0 / cf->E0 / MBT_FGIGA;
+          double dTau = cf->dT * cf->sigma_tau;
+          double tbucket = (cf->T0 / cf->h - Nc * dTau);
+          L.fac = Amp / (unsigned)cf->Np * MBT_FMILLI;
+          double C0 = -alpha, C1 = res->wr;
+          L.pr0 = exp(C0 * dTau) * cos(C1 * dTau);
+          L.pr1 = exp(C0 * dTau) * sin(C1 * dTau);
+          L.ge = exp(C0 * tbucket);
+          L.gc = cos(C1 * tbucket);
+          L.gs = sin(C1 * tbucket);
+          double tb = cf->T0 / cf->h;
+          L.ee = exp(C0 * tb);
+          L.ec = cos(C1 * tb);
+          L.es = sin(C1 * tb);
+          L.fb = cf->has_rf_feedback && p == 0 && cf->rffb_resonator == l + 1;
+          double taubeam = 0, taub2beam = 0;
+          if (L.fb) {
+            taubeam = cf->Nsigma * cf->sigma_tau + dTau * 1.5;
+            taub2beam = -cf->Nsigma * cf->sigma_tau + dTau * 0.5 - tbucket;
+          }
+          L.pb0 = exp(C0 * taubeam) * cos(C1 * taubeam);
+          L.pb1 = exp(C0 * taubeam) * sin(C1 * taubeam);
+          L.pa0 = exp(C0 * taub2beam) * cos(C1 * taub2beam);
+          L.pa1 = exp(C0 * taub2beam) * sin(C1 * taub2beam);
+        }
+      }
     }
   }
+  lrf[R] = nres;
   r->nres = nres;
   DA(r->lc, nres > 0 ? nres : 1);
-  DA(r->lrc, (size_t)(nres > 0 ? nres : 1) * h * NcA);
+  DA(r->lr_first, R + 1);
+  UP(r->lr_first, lrf, R + 1);
+  DA(r->lrc, lrc_total > 0 ? lrc_total : 1);
   DA(r->phasor_end, (size_t)2 * (nres > 0 ? nres : 1));
-  r->fb_len = cf->has_rf_feedback ? abs(cf->rffb_len_average) : 1;
-  r->fb_off = cf->has_rf_feedback && cf->rffb_len_average < 0;
-  DA(r->fb_hist, (size_t)2 * r->fb_len);
   // ---- uploads
-  CK(hipMemcpy(r->wg_first, wf, r->nwg * sizeof(long long), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->wg_end, we, r->nwg * sizeof(long long), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->wg_bunch, wb, r->nwg * sizeof(int), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->bunch_wg0, b0, (Nb + 1) * sizeof(int), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->np, nph, Nb * sizeof(int), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->bucket, cf->bucket, Nb * sizeof(int), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->nfFill, cf->nfFill, h * sizeof(int), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->Ibunch, cf->Ibunch, h * sizeof(double), hipMemcpyHostToDevice));
-  CK(hipMemcpy(r->bunch_Ib, cf->bunch_Ib, Nb * sizeof(double), hipMemcpyHostToDevice));
-  if (Nc > 0) {
-    CK(hipMemcpy(r->Gl, Gl, Nc * sizeof(double), hipMemcpyHostToDevice));
-    CK(hipMemcpy(r->Gv, Gv, Nc * sizeof(double), hipMemcpyHostToDevice));
-    CK(hipMemcpy(r->Gh, Gh, Nc * sizeof(double), hipMemcpyHostToDevice));
+  UP(r->wg_first, wf, nwg);
+  UP(r->wg_end, we, nwg);
+  UP(r->wg_bunch, wb, nwg);
+  UP(r->bunch_wg0, b0, G + 1);
+  UP(r->bunch_first, bfirst, G);
+  UP(r->np, nph, G);
+  UP(r->bunch_ring, bring, G);
+  UP(r->bucket, bk, G);
+  UP(r->bucket_ring, qring, H);
+  UP(r->nfFill, fill, H);
+  UP(r->Ibunch, Ibq, H);
+  UP(r->bunch_Ib, bIb, G);
+  UP(r->Gf, Gf, (size_t)R * 3 * NcA);
+  UP(r->P, r->Ph, R);
+  if (nres > 0) UP(r->lc, lc, nres);
+  {
+    int lp[3] = {0, 0, 0};
+    for (int k = 0; k < R; k++)
+      for (int p = 0; p < 3; p++) lp[p] |= r->rh[k].lr_plane[p];
+    UP(r->lr_planes, lp, 3);
   }
-  if (nres > 0) CK(hipMemcpy(r->lc, lc, nres * sizeof(LRConst), hipMemcpyHostToDevice));
-  if (cf->EnableRW_long) {
-    double *st = (double *)malloc((M + 1) * sizeof(double));
-    double deltab = cf->T0 / cf->h;
-    st[0] = 1.0;
-    for (long long m = 1; m <= M; m++) st[m] = sqrt(m * deltab);
-    CK(hipMemcpy(r->sqrt_tt, st, (M + 1) * sizeof(double), hipMemcpyHostToDevice));
+  if (st_total > 0) {
+    double *st = (double *)malloc(st_total * sizeof(double));
+    for (int k = 0; k < R; k++) {
+      const mbt_config *cf = &r->rh[k].cfg;
+      if (!cf->EnableRW_long) continue;
+      long long M = (long long)(cf->Nmlt + 1) * cf->Nharm;
+      double deltab = cf->T0 / cf->h;
+      double *s = st + r->Ph[k].st_off;
+      s[0] = 1.0;
+      for (long long m = 1; m <= M; m++) s[m] = sqrt(m * deltab);
+    }
+    UP(r->sqrt_tt, st, st_total);
+    CK(hipStreamSynchronize(r->stream));
     free(st);
   }
-  TurnState ts0;
-  memset(&ts0, 0, sizeof ts0);
-  ts0.phai0 = cf->phai0;
-  ts0.Vrf0 = cf->Vrf0;
-  ts0.Iring = cf->Iring;
-  ts0.scan_val = cf->scan_start;
-  CK(hipMemcpy(r->ts, &ts0, sizeof ts0, hipMemcpyHostToDevice));
+  {
+    TurnState *ts0 = (TurnState *)calloc(R, sizeof(TurnState));
+    for (int k = 0; k < R; k++) {
+      const mbt_config *cf = &r->rh[k].cfg;
+      ts0[k].phai0 = cf->phai0;
+      ts0[k].Vrf0 = cf->Vrf0;
+      ts0[k].phai0_prev = cf->phai0;
+      ts0[k].Vrf0_prev = cf->Vrf0;
+      ts0[k].Iring = cf->Iring;
+      ts0[k].scan_val = cf->scan_start;
+    }
+    UP(r->ts, ts0, R);
+    CK(hipStreamSynchronize(r->stream));
+    free(ts0);
+  }
   free(wf);
   free(we);
   free(wb);
   free(b0);
   free(nph);
-  free(Gl);
-  free(Gv);
-  free(Gh);
-  r->R.stats = r->stats;
-  r->R.amp = r->amp;
-  r->R.shift = r->shift;
-  r->R.rwhist = r->rwhist;
-  r->R.fnp_ring = r->fnp_ring;
-  r->R.sfL = r->sfL;
-  r->R.sfV = r->sfV;
-  r->R.sfH = r->sfH;
-  r->R.fnp_out = r->fnp_out;
-  r->R.dipV_out = r->dipV_out;
-  r->R.dipH_out = r->dipH_out;
-  r->R.trash_tot = r->trash_tot;
-  r->R.trash_pend = r->trash_pend;
-  r->R.Gl = r->Gl;
-  r->R.Gv = r->Gv;
-  r->R.Gh = r->Gh;
-  r->R.bunch_Ib = r->bunch_Ib;
-  r->R.np = r->np;
+  free(bring);
+  free(bk);
+  free(bfirst);
+  free(qring);
+  free(fill);
+  free(Ibq);
+  free(bIb);
+  free(Gf);
+  free(lc);
+  free(lrf);
+  RedOut &Ro = r->Ro;
+  Ro.stats = r->stats;
+  Ro.amp = r->amp;
+  Ro.shift = r->shift;
+  Ro.rwhist = r->rwhist;
+  Ro.fnp_ring = r->fnp_ring;
+  Ro.plane_stride = (long long)H * NcA;
+  Ro.sfL = r->sfL;
+  Ro.sfV = r->sfV;
+  Ro.sfH = r->sfH;
+  Ro.fnp_out = r->fnp_out;
+  Ro.dipV_out = r->dipV_out;
+  Ro.dipH_out = r->dipH_out;
+  Ro.trash_tot = r->trash_tot;
+  Ro.trash_pend = r->trash_pend;
+  Ro.G = r->Gf;
+  Ro.bunch_Ib = r->bunch_Ib;
+  Ro.np = r->np;
+  Ro.Gtot = G;
+  Ro.rfbuf = r->rfbuf;
+  Ro.R = R;
+  r->use_graphs = !(flags & MBT_RING_NO_GRAPHS);
   r->lds_turn = turn_lds(Nc);
   r->lds_red = (size_t)3 * NcA * 8 + 64;
   if (r->lds_turn > 64 * 1024) {
@@ -1435,49 +1705,93 @@ extern "C" int mbt_ring_create(const mbt_config *c, int device, int flags, mbt_r
   }
   // ---- initial distributions (host threads, one glibc stream per bunch)
   if (!(flags & MBT_RING_NO_INIT_DIST)) {
-    double *buf = (double *)malloc(sizeof(double) * 6 * (size_t)Np * Nb);
-    if (!buf) {
-      mbt_ring_destroy(r);
-      return MBT_ENOMEM;
+    for (int k = 0; k < R; k++) {
+      const mbt_config *cf = &r->rh[k].cfg;
+      const size_t per = (size_t)6 * cf->Np;
+      double *buf = (double *)malloc(sizeof(double) * per * cf->Nbunch);
+      if (!buf) {
+        mbt_ring_destroy(r);
+        return MBT_ENOMEM;
+      }
+      int rc = mbt_generate_all(cf, buf);
+      for (int b = 0; b < cf->Nbunch && rc == MBT_OK; b++)
+        rc = mbt_ensemble_set_bunch(r, r->Ph[k].bunch_off + b, buf + per * b);
+      free(buf);
+      if (rc) {
+        mbt_ring_destroy(r);
+        return rc;
+      }
     }
-    rc = mbt_generate_all(cf, buf);
-    for (int b = 0; b < Nb && rc == MBT_OK; b++) rc = mbt_ring_set_bunch(r, b, buf + (size_t)6 * Np * b);
-    free(buf);
-    if (rc) {
-      mbt_ring_destroy(r);
-      return rc;
-    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    mbt_set_error("device synchronisation after set-up failed");
+    mbt_ring_destroy(r);
+    return MBT_EDEVICE;
   }
   *out = r;
   return MBT_OK;
 }
 
-extern "C" int mbt_ring_set_bunch(mbt_ring *r, int ib, const double *s)
+extern "C" int mbt_ring_create(const mbt_config *c, int device, int flags, mbt_ring **out)
 {
-  if (ib < 0 || ib >= r->Nb) return MBT_EINVAL;
+  const mbt_config *one[1] = {c};
+  return create_rings(one, 1, device, flags, out);
+}
+
+extern "C" int mbt_ensemble_create(const mbt_config *const *cfgs, int nrings, int device, int flags, mbt_ring **out)
+{
+  return create_rings(cfgs, nrings, device, flags, out);
+}
+
+static long long bunch_first_host(const mbt_ring *r, int g, int *np)
+{
+  long long pos = 0;
+  for (int k = 0; k < r->R; k++) {
+    const mbt_config *c = &r->rh[k].cfg;
+    if (g < c->Nbunch) {
+      *np = c->Np;
+      return pos + (long long)g * c->Np;
+    }
+    g -= c->Nbunch;
+    pos += (long long)c->Nbunch * c->Np;
+  }
+  *np = 0;
+  return -1;
+}
+
+extern "C" int mbt_ensemble_set_bunch(mbt_ring *r, int g, const double *s)
+{
+  if (g < 0 || g >= r->G) return MBT_EINVAL;
   CK(hipSetDevice(r->device));
-  const long long Np = r->cfg.Np;
+  int Np;
+  const long long first = bunch_first_host(r, g, &Np);
   double *dst[6] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp};
   for (int c = 0; c < 6; c++)
-    CK(hipMemcpy(dst[c] + ib * Np, s + c * Np, Np * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpyAsync(dst[c] + first, s + (size_t)c * Np, Np * sizeof(double), hipMemcpyHostToDevice, r->stream));
+  CK(hipStreamSynchronize(r->stream));
   return MBT_OK;
 }
 
-extern "C" int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *s)
+extern "C" int mbt_ensemble_get_bunch(const mbt_ring *r, int g, double *s)
 {
-  if (ib < 0 || ib >= r->Nb) return MBT_EINVAL;
+  if (g < 0 || g >= r->G) return MBT_EINVAL;
   CK(hipSetDevice(r->device));
   CK(hipStreamSynchronize(r->stream));
-  const long long Np = r->cfg.Np;
+  int Np;
+  const long long first = bunch_first_host(r, g, &Np);
   double *src[6] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp};
   for (int c = 0; c < 6; c++)
-    CK(hipMemcpy(s + c * Np, src[c] + ib * Np, Np * sizeof(double), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(s + (size_t)c * Np, src[c] + first, Np * sizeof(double), hipMemcpyDeviceToHost));
   return MBT_OK;
 }
 
-typedef void (*turn_kernel_t)(DevParams, Coords, const long long *, const long long *, const int *, const int *,
-                              const double *, const double *, const double *, const double *, const double *,
-                              const double *, const double *, const double *, double *, HistPtr, const TurnState *);
+extern "C" int mbt_ring_set_bunch(mbt_ring *r, int ib, const double *s) { return mbt_ensemble_set_bunch(r, ib, s); }
+extern "C" int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *s) { return mbt_ensemble_get_bunch(r, ib, s); }
+
+typedef void (*turn_kernel_t)(const DevParams *, Coords, const long long *, const long long *, const int *,
+                              const int *, const int *, const long long *, const double *, const double *,
+                              const double *, const double *, const double *, long long, const double *, double *,
+                              HistPtr, const TurnState *);
 
 #define TK(Q, D, G, M) k_turn<MODE_TURN, Q, D, G, M>
 static const turn_kernel_t g_turn_kernels[16] = {
@@ -1495,48 +1809,44 @@ static int launch_turn_pass(mbt_ring *r, int mode)
   if (mode == MODE_STATS) k = k_turn<MODE_STATS, false, true, true, false>;
   else if (mode == MODE_FINAL) k = k_turn<MODE_FINAL, false, true, true, false>;
   else {
-    const bool mon = ((r->rev_host + 1) % r->cfg.NrevMon) == 0;
-    const bool dip = r->P.kickV || r->P.kickH || r->lr_plane[1] || r->lr_plane[2];
-    const bool gen = r->P.ideal_hc || r->P.n_ahc > 0 || r->P.amp_dq;
-    k = g_turn_kernels[(r->P.qe ? 8 : 0) + (dip ? 4 : 0) + (gen ? 2 : 0) + (mon ? 1 : 0)];
+    const bool mon = ((r->rev_host + 1) % r->nrev_mon) == 0;
+    k = g_turn_kernels[(r->any_qe ? 8 : 0) + (r->any_dip ? 4 : 0) + (r->any_gen ? 2 : 0) + (mon ? 1 : 0)];
   }
-  const size_t NcA = r->Nc > 0 ? r->Nc : 1;
+  const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
   hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
-                     r->wg_bunch, r->bucket, r->shift, r->sfL, r->sfV, r->sfH, r->lr_wake,
-                     r->lr_wake + (size_t)r->h * NcA, r->lr_wake + (size_t)2 * r->h * NcA, r->pending, r->partial,
-                     r->H, r->ts);
+                     r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
+                     r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
   CK(hipGetLastError());
   return MBT_OK;
 }
 
 static int launch_reduce(mbt_ring *r, int phase)
 {
-  hipLaunchKernelGGL(k_reduce, dim3(r->Nb), dim3(WG), r->lds_red, r->stream, r->P, r->partial, r->bunch_wg0, r->bucket,
-                     r->H, r->R, r->ts, phase, r->lr_plane[0], r->lr_plane[1], r->lr_plane[2]);
+  hipLaunchKernelGGL(k_reduce, dim3(r->G), dim3(WG), r->lds_red, r->stream, r->P, r->partial, r->bunch_wg0,
+                     r->bunch_ring, r->bucket, r->Hs, r->Ro, r->ts, phase, r->lr_planes);
   CK(hipGetLastError());
   return MBT_OK;
 }
 
-static int launch_lr(mbt_ring *r, int init_turns)
+static int launch_lr(mbt_ring *r, int init)
 {
   if (r->nres == 0) return MBT_OK;
-  hipLaunchKernelGGL(k_lr, dim3(1), dim3(64), 0, r->stream, r->nres, r->lc, r->fnp_ring, r->Ibunch, r->nfFill, r->h,
-                     r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->fb_len, r->ts, init_turns);
+  const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
+  hipLaunchKernelGGL(k_lr, dim3((r->nres + 63) / 64), dim3(64), 0, r->stream, r->nres, r->lc, r->P, r->fnp_ring,
+                     stride, r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
   CK(hipGetLastError());
-  if (!init_turns) {
-    long long tot = 3LL * r->h * r->Nc;
-    hipLaunchKernelGGL(k_lr_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->nres, r->lc, r->lrc,
-                       r->nfFill, r->h, r->Nc, r->lr_wake);
+  if (!init) {
+    long long tot = 3LL * r->H * r->Nc;
+    hipLaunchKernelGGL(k_lr_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->lr_first, r->lc, r->lrc,
+                       r->nfFill, r->bucket_ring, r->H, r->Nc, r->lr_wake);
     CK(hipGetLastError());
   }
   return MBT_OK;
 }
 
-static int launch_fb_solve(mbt_ring *r, int switch_off)
+static int launch_fb_solve(mbt_ring *r, int init)
 {
-  if (!r->cfg.has_rf_feedback || r->nres == 0) return MBT_OK;
-  hipLaunchKernelGGL(k_fb_solve, dim3(1), dim3(64), 0, r->stream, r->ts, r->fb_hist, r->fb_len, r->cfg.E0,
-                     r->cfg.rffb_vrf_design, r->cfg.rffb_phi0_design, switch_off);
+  hipLaunchKernelGGL(k_fb_solve, dim3((r->R + 63) / 64), dim3(64), 0, r->stream, r->R, r->P, r->ts, r->fb_hist, init);
   CK(hipGetLastError());
   return MBT_OK;
 }
@@ -1545,26 +1855,25 @@ static int launch_fb_solve(mbt_ring *r, int switch_off)
 static int pre_turn(mbt_ring *r)
 {
   int rc;
-  if (r->P.sf_active && (rc = launch_lr(r, 0))) return rc;
-  hipLaunchKernelGGL(k_fb_solve, dim3(1), dim3(64), 0, r->stream, r->ts, r->fb_hist, r->fb_len, r->cfg.E0,
-                     r->cfg.rffb_vrf_design, r->cfg.rffb_phi0_design, 0);
-  CK(hipGetLastError());
-  return MBT_OK;
+  if (r->sf_any && (rc = launch_lr(r, 0))) return rc;
+  return launch_fb_solve(r, 0);
 }
 
 static int download_slots(mbt_ring *r, long long rev0, long n, double *stats, double *ampinv, double *rf)
 {
   CK(hipStreamSynchronize(r->stream));
-  const int Nb = r->Nb;
+  const int G = r->G;
   for (long t = 0; t < n; t++) {
     int slot = (int)((rev0 + t) % STATS_TURNS);
     if (stats)
-      CK(hipMemcpy(stats + (size_t)t * Nb * 12, r->stats + (size_t)slot * Nb * 12, Nb * 12 * sizeof(double),
+      CK(hipMemcpy(stats + (size_t)t * G * 12, r->stats + (size_t)slot * G * 12, (size_t)G * 12 * sizeof(double),
                    hipMemcpyDeviceToHost));
-    if (ampinv && ((rev0 + t + 1) % r->cfg.NrevMon == 0))
-      CK(hipMemcpy(ampinv + (size_t)t * Nb * 6, r->amp + (size_t)slot * Nb * 6, Nb * 6 * sizeof(double),
+    if (ampinv && ((rev0 + t + 1) % r->nrev_mon == 0))
+      CK(hipMemcpy(ampinv + (size_t)t * G * 6, r->amp + (size_t)slot * G * 6, (size_t)G * 6 * sizeof(double),
                    hipMemcpyDeviceToHost));
-    if (rf) CK(hipMemcpy(rf + 2 * t, r->rfbuf + 2 * slot, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (rf)
+      CK(hipMemcpy(rf + (size_t)2 * r->R * t, r->rfbuf + (size_t)2 * r->R * slot, 2 * r->R * sizeof(double),
+                   hipMemcpyDeviceToHost));
   }
   return MBT_OK;
 }
@@ -1574,7 +1883,7 @@ extern "C" int mbt_ring_prepare(mbt_ring *r, double *init_stats)
   CK(hipSetDevice(r->device));
   int rc;
   // two stats passes: means (shift 0), then moments about the means + histograms
-  CK(hipMemsetAsync(r->shift, 0, (size_t)r->Nb * 6 * sizeof(double), r->stream));
+  CK(hipMemsetAsync(r->shift, 0, (size_t)r->G * 6 * sizeof(double), r->stream));
   if ((rc = launch_turn_pass(r, MODE_STATS))) return rc;
   if ((rc = launch_reduce(r, 0))) return rc;
   if ((rc = launch_turn_pass(r, MODE_STATS))) return rc;
@@ -1582,26 +1891,105 @@ extern "C" int mbt_ring_prepare(mbt_ring *r, double *init_stats)
   if ((rc = launch_reduce(r, 1))) return rc;
   if (init_stats) {
     CK(hipStreamSynchronize(r->stream));
-    CK(hipMemcpy(init_stats, r->stats, (size_t)r->Nb * 12 * sizeof(double), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(init_stats, r->stats, (size_t)r->G * 12 * sizeof(double), hipMemcpyDeviceToHost));
   }
   // LR resonators: wake_phasor_init replays, then the RF-feedback design solve
   if (r->nres > 0) {
-    TurnState ts;
+    TurnState *ts = (TurnState *)malloc(r->R * sizeof(TurnState));
     CK(hipStreamSynchronize(r->stream));
-    CK(hipMemcpy(&ts, r->ts, sizeof ts, hipMemcpyDeviceToHost));
-    ts.has_fb = r->cfg.has_rf_feedback ? 1 : 0;
-    ts.fb_count = 0;
-    CK(hipMemcpy(r->ts, &ts, sizeof ts, hipMemcpyHostToDevice));
-    int nturns = (int)(r->cfg.Nbumax / r->cfg.h);
-    if ((rc = launch_lr(r, nturns > 0 ? nturns : 0))) return rc;
-    if (nturns <= 0) {
-      // no replay: phasors stay at zero (Nbumax < h)
+    CK(hipMemcpy(ts, r->ts, r->R * sizeof(TurnState), hipMemcpyDeviceToHost));
+    for (int k = 0; k < r->R; k++) {
+      ts[k].has_fb = r->rh[k].cfg.has_rf_feedback ? 1 : 0;
+      ts[k].fb_count = 0;
     }
-    if ((rc = launch_fb_solve(r, r->fb_off))) return rc;
+    CK(hipMemcpyAsync(r->ts, ts, r->R * sizeof(TurnState), hipMemcpyHostToDevice, r->stream));
+    CK(hipStreamSynchronize(r->stream));
+    free(ts);
+    // each resonator replays its ring's Nbumax / h turns (none when that is 0)
+    if ((rc = launch_lr(r, 1))) return rc;
+    if ((rc = launch_fb_solve(r, 1))) return rc;
   }
   if ((rc = pre_turn(r))) return rc;
   CK(hipStreamSynchronize(r->stream));
   r->rev_host = 0;
+  return MBT_OK;
+}
+
+// One turn's launches.  The reference runs the long-range wakes and the
+// RF-feedback solve at the start of a turn; they run here at the end of the
+// previous one (nothing changes their inputs in between).  With E, events
+// E[0..4] bracket the kernel classes for per-kernel timing.
+static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
+{
+  int rc;
+  const int lr_launch = r->sf_any && r->nres > 0;
+  if (E) CK(hipEventRecord(E[0], r->stream));
+  if ((rc = launch_turn_pass(r, MODE_TURN))) return rc;
+  if (E) CK(hipEventRecord(E[1], r->stream));
+  if ((rc = launch_reduce(r, 2))) return rc;
+  if (E) CK(hipEventRecord(E[2], r->stream));
+  if (r->rw_total > 0)
+    hipLaunchKernelGGL(k_rw, dim3(r->G), dim3(WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
+                       r->Ibunch, r->nfFill, r->bucket, r->pending, r->ts, r->ring_done, r->bunch_Ib, !lr_launch);
+  else
+    hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
+                       !lr_launch);
+  CK(hipGetLastError());
+  if (E) CK(hipEventRecord(E[3], r->stream));
+  if (lr_launch) {
+    if ((rc = launch_lr(r, 0))) return rc;
+    if ((rc = launch_fb_solve(r, 0))) return rc;
+  }
+  if (E) CK(hipEventRecord(E[4], r->stream));
+  r->rev_host++;
+  return MBT_OK;
+}
+
+// n turns from the current turn as one HIP graph.  The launches of a turn
+// depend on the turn number only through the monitor-turn variant of k_turn,
+// so a graph is reusable for any block of n turns at the same phase modulo
+// NrevMon; a few are cached.  With timed, the graph also records the timing
+// events of every turn (event record nodes).
+static int turn_graph(mbt_ring *r, long n, int timed, hipGraphExec_t *out)
+{
+  const long long phase = r->rev_host % r->nrev_mon;
+  for (int i = 0; i < NGRAPH; i++)
+    if (r->graphs[i].exec && r->graphs[i].phase == phase && r->graphs[i].n == n && r->graphs[i].timed == timed) {
+      *out = r->graphs[i].exec;
+      return MBT_OK;
+    }
+  GraphEntry &g = r->graphs[r->graph_next];
+  r->graph_next = (r->graph_next + 1) % NGRAPH;
+  if (g.exec) {
+    CK(hipGraphExecDestroy(g.exec));
+    g.exec = nullptr;
+  }
+  const long long rev_save = r->rev_host;
+  CK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+  int rc = MBT_OK;
+  for (long t = 0; t < n && rc == MBT_OK; t++) rc = enqueue_turn(r, timed ? r->evpool + 6 * t : NULL);
+  r->rev_host = rev_save;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(r->stream, &graph);
+  if (rc != MBT_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    mbt_set_error("turn graph capture failed: %s", hipGetErrorString(e));
+    return MBT_EDEVICE;
+  }
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g.exec = nullptr;
+    mbt_set_error("turn graph instantiation failed: %s", hipGetErrorString(e));
+    return MBT_EDEVICE;
+  }
+  g.phase = phase;
+  g.n = n;
+  g.timed = timed;
+  *out = g.exec;
   return MBT_OK;
 }
 
@@ -1613,55 +2001,43 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
     r->kt_ms[i] = 0;
     r->kt_launches[i] = 0;
   }
+  // blocks of turns whose statistics fit the device ring buffer; a multiple of
+  // NrevMon where possible, so that every full block replays the same graph
+  long block = STATS_TURNS;
+  if (r->nrev_mon <= STATS_TURNS) block -= STATS_TURNS % r->nrev_mon;
   long done = 0;
   while (done < nturns) {
     long n = nturns - done;
-    if (n > STATS_TURNS) n = STATS_TURNS;
+    if (n > block) n = block;
     long long rev0 = r->rev_host;
     const bool timed = r->timing != 0;
-    for (long t = 0; t < n; t++) {
-      hipEvent_t *E = timed ? r->evpool + 6 * t : NULL;
-      hipLaunchKernelGGL(k_record_rf, dim3(1), dim3(64), 0, r->stream, r->ts, r->rfbuf);
-      if (timed) CK(hipEventRecord(E[1], r->stream));
-      if ((rc = launch_turn_pass(r, MODE_TURN))) return rc;
-      if (timed) CK(hipEventRecord(E[2], r->stream));
-      if ((rc = launch_reduce(r, 2))) return rc;
-      if (timed) CK(hipEventRecord(E[3], r->stream));
-      if (r->P.rw) {
-        hipLaunchKernelGGL(k_rw, dim3(r->Nb), dim3(WG), 0, r->stream, r->P, r->rwhist, r->sqrt_tt, r->Ibunch,
-                           r->nfFill, r->bucket, r->pending, r->ts);
-        CK(hipGetLastError());
-      }
-      if (timed) CK(hipEventRecord(E[4], r->stream));
-      hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
-                         r->cfg.scan, (long long)r->cfg.NrevScan, r->cfg.scan_step);
-      CK(hipGetLastError());
-      // the next turn's long-range wakes and RF-feedback solve (the reference runs
-      // them at the start of the turn; nothing changes the inputs in between)
-      if (timed) CK(hipEventRecord(E[0], r->stream));
-      if ((rc = pre_turn(r))) return rc;
-      if (timed) CK(hipEventRecord(E[5], r->stream));
-      r->rev_host++;
+    if (!r->use_graphs) {
+      for (long t = 0; t < n; t++)
+        if ((rc = enqueue_turn(r, timed ? r->evpool + 6 * t : NULL))) return rc;
+    } else {
+      hipGraphExec_t exec;
+      if ((rc = turn_graph(r, n, timed, &exec))) return rc;
+      CK(hipGraphLaunch(exec, r->stream));
+      r->rev_host += n;
     }
     if (timed) {
       CK(hipStreamSynchronize(r->stream));
       for (long t = 0; t < n; t++) {
         hipEvent_t *E = r->evpool + 6 * t;
         float a, b, c, d;
-        CK(hipEventElapsedTime(&a, E[0], E[5]));
+        CK(hipEventElapsedTime(&a, E[0], E[1]));
         CK(hipEventElapsedTime(&b, E[1], E[2]));
         CK(hipEventElapsedTime(&c, E[2], E[3]));
         CK(hipEventElapsedTime(&d, E[3], E[4]));
-        (void)0;
-        r->kt_ms[2] += a;
-        r->kt_ms[0] += b;
-        r->kt_ms[1] += c;
-        r->kt_ms[3] += d;
+        r->kt_ms[0] += a;
+        r->kt_ms[1] += b;
+        r->kt_ms[3] += c;
+        r->kt_ms[2] += d;
         for (int i = 0; i < 4; i++) r->kt_launches[i]++;
       }
     }
-    rc = download_slots(r, rev0, n, stats ? stats + (size_t)done * r->Nb * 12 : NULL,
-                        ampinv ? ampinv + (size_t)done * r->Nb * 6 : NULL, rf ? rf + 2 * done : NULL);
+    rc = download_slots(r, rev0, n, stats ? stats + (size_t)done * r->G * 12 : NULL,
+                        ampinv ? ampinv + (size_t)done * r->G * 6 : NULL, rf ? rf + (size_t)2 * r->R * done : NULL);
     if (rc) return rc;
     done += n;
   }
@@ -1675,18 +2051,17 @@ extern "C" int mbt_ring_final_stats(mbt_ring *r, double *stats)
   int rc;
   // apply the pending RW kick, then two exact passes for the statistics
   if ((rc = launch_turn_pass(r, MODE_FINAL))) return rc;
-  CK(hipMemsetAsync(r->pending, 0, (size_t)r->Nb * 2 * sizeof(double), r->stream));
-  CK(hipMemsetAsync(r->shift, 0, (size_t)r->Nb * 6 * sizeof(double), r->stream));
+  CK(hipMemsetAsync(r->pending, 0, (size_t)r->G * 2 * sizeof(double), r->stream));
+  CK(hipMemsetAsync(r->shift, 0, (size_t)r->G * 6 * sizeof(double), r->stream));
   if ((rc = launch_turn_pass(r, MODE_STATS))) return rc;
   if ((rc = launch_reduce(r, 0))) return rc;
   if ((rc = launch_turn_pass(r, MODE_STATS))) return rc;
-  // stats land in the slot of the current turn counter; histograms stay for the next turn
   if ((rc = launch_reduce(r, 3))) return rc;
   CK(hipStreamSynchronize(r->stream));
   TurnState ts;
   CK(hipMemcpy(&ts, r->ts, sizeof ts, hipMemcpyDeviceToHost));
   int slot = (int)(ts.rev % STATS_TURNS);
-  CK(hipMemcpy(stats, r->stats + (size_t)slot * r->Nb * 12, (size_t)r->Nb * 12 * sizeof(double),
+  CK(hipMemcpy(stats, r->stats + (size_t)slot * r->G * 12, (size_t)r->G * 12 * sizeof(double),
                hipMemcpyDeviceToHost));
   return MBT_OK;
 }
@@ -1695,27 +2070,25 @@ extern "C" int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash
 {
   CK(hipSetDevice(r->device));
   CK(hipStreamSynchronize(r->stream));
-  if (r->Nc > 0) CK(hipMemcpy(counts, r->fnp_out, (size_t)r->Nb * r->Nc * sizeof(double), hipMemcpyDeviceToHost));
-  int *tt = (int *)malloc((size_t)r->Nb * 2 * sizeof(int)), *tp = (int *)malloc((size_t)r->Nb * 2 * sizeof(int));
-  hipMemcpy(tt, r->trash_tot, (size_t)r->Nb * 2 * sizeof(int), hipMemcpyDeviceToHost);
-  hipMemcpy(tp, r->trash_pend, (size_t)r->Nb * 2 * sizeof(int), hipMemcpyDeviceToHost);
-  for (int i = 0; i < 2 * r->Nb; i++) trash[i] = tt[i] + (r->P.sf_active ? 0 : 0);
-  free(tt);
-  free(tp);
+  if (r->Nc > 0) CK(hipMemcpy(counts, r->fnp_out, (size_t)r->G * r->Nc * sizeof(double), hipMemcpyDeviceToHost));
+  CK(hipMemcpy(trash, r->trash_tot, (size_t)r->G * 2 * sizeof(int), hipMemcpyDeviceToHost));
   return MBT_OK;
 }
 
 extern "C" int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, double *rfprev)
 {
-  if (ib < 0 || ib >= r->Nb || r->Nc <= 0) return MBT_EINVAL;
+  if (ib < 0 || ib >= r->G || r->Nc <= 0) return MBT_EINVAL;
   CK(hipSetDevice(r->device));
   CK(hipStreamSynchronize(r->stream));
-  const size_t Nc = r->Nc, off = (size_t)ib * Nc, loff = (size_t)r->cfg.bucket[ib] * Nc;
+  int ring = 0, local = ib;
+  while (local >= r->rh[ring].cfg.Nbunch) local -= r->rh[ring++].cfg.Nbunch;
+  const size_t Nc = r->Nc, off = (size_t)ib * Nc;
+  const size_t loff = (size_t)(r->Ph[ring].bucket_off + r->rh[ring].cfg.bucket[local]) * Nc;
   const double *src[7] = {r->sfL + off, r->lr_wake + loff, r->fnp_out + off, r->sfV + off, r->dipV_out + off,
                           r->sfH + off, r->dipH_out + off};
   for (int k = 0; k < 7; k++) CK(hipMemcpy(out + k * Nc, src[k], Nc * sizeof(double), hipMemcpyDeviceToHost));
   TurnState ts;
-  CK(hipMemcpy(&ts, r->ts, sizeof ts, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(&ts, r->ts + ring, sizeof ts, hipMemcpyDeviceToHost));
   if (rfprev) {
     rfprev[0] = ts.phai0_prev;
     rfprev[1] = ts.Vrf0_prev;
@@ -1726,10 +2099,11 @@ extern "C" int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, doubl
 
 extern "C" int mbt_ring_rw_history(const mbt_ring *r, double *out)
 {
-  if (!r->cfg.EnableRW_long) return MBT_EINVAL;
+  if (!r->rh[0].cfg.EnableRW_long) return MBT_EINVAL;
   CK(hipSetDevice(r->device));
   CK(hipStreamSynchronize(r->stream));
-  CK(hipMemcpy(out, r->rwhist, (size_t)(r->cfg.Nmlt + 2) * r->h * 2 * sizeof(double), hipMemcpyDeviceToHost));
+  const mbt_config *c = &r->rh[0].cfg;
+  CK(hipMemcpy(out, r->rwhist, (size_t)(c->Nmlt + 2) * c->Nharm * 2 * sizeof(double), hipMemcpyDeviceToHost));
   return MBT_OK;
 }
 
@@ -1744,6 +2118,13 @@ extern "C" int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launch
 
 extern "C" double mbt_ring_bytes_per_turn(const mbt_ring *r) { return 96.0 * (double)r->Ntot; }
 extern "C" long mbt_ring_turn(const mbt_ring *r) { return (long)r->rev_host; }
+extern "C" int mbt_ensemble_size(const mbt_ring *r, int *nrings, int *nbunch_total, long long *nparticles)
+{
+  if (nrings) *nrings = r->R;
+  if (nbunch_total) *nbunch_total = r->G;
+  if (nparticles) *nparticles = r->Ntot;
+  return MBT_OK;
+}
 
 extern "C" int mbt_ring_set_timing(mbt_ring *r, int on)
 {

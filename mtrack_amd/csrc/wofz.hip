@@ -39,7 +39,7 @@ static int run_wofz(int device, long n, const T *x, const T *y, T *wr, T *wi, in
   int *df = NULL;
   size_t bytes = (size_t)n * sizeof(T);
   if (hipMalloc((void **)&d, 4 * bytes) != hipSuccess || hipMalloc((void **)&df, n * sizeof(int)) != hipSuccess) {
-    if (d) hipFree(d);
+    if (d) (void)hipFree(d);
     mbt_set_error("hipMalloc failed");
     return MBT_ENOMEM;
   }
@@ -56,8 +56,8 @@ static int run_wofz(int device, long n, const T *x, const T *y, T *wr, T *wi, in
                        hipMemcpy(wi, d + 3 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
                        hipMemcpy(flag, df, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
     rc = MBT_EDEVICE;
-  hipFree(d);
-  hipFree(df);
+  (void)hipFree(d);
+  (void)hipFree(df);
   if (rc) mbt_set_error("mbt_wofz: HIP failure");
   return rc;
 }

@@ -30,3 +30,23 @@ def test_error_reporting():
         assert "cannot open" in str(e)
     else:
         raise AssertionError("expected failure")
+
+
+def test_ensemble_argument_checks(tmp_path):
+    """Rejected before any device call: rings of one ensemble share the slicing."""
+    from variants import write_variant
+
+    a = m.Config(write_variant("wakes", str(tmp_path / "a"), selffield={"Ncell": 40}), str(tmp_path / "a") + "/")
+    b = m.Config(write_variant("rsis", str(tmp_path / "b"), selffield={"Ncell": 60}), str(tmp_path / "b") + "/")
+    try:
+        m.Ensemble([a, b])
+    except m.MbtError as e:
+        assert "share Ncell" in str(e)
+    else:
+        raise AssertionError("expected failure")
+    try:
+        m.Ensemble([])
+    except m.MbtError as e:
+        assert "no ring" in str(e)
+    else:
+        raise AssertionError("expected failure")

@@ -70,10 +70,17 @@ VARIANTS = {
 FILL_BUCKETS = [int(k * 6.5) for k in range(64)]
 
 
-def write_variant(name, d):
-    """Writes the variant's .conf (and filling file) into directory d; returns the path."""
+def write_variant(name, d, **section_overrides):
+    """Writes the variant's .conf (and filling file) into directory d; returns the
+    path.  section_overrides maps a section name to keys that replace the
+    variant's own (e.g. selffield={"Ncell": 40})."""
     os.makedirs(d, exist_ok=True)
     fill = os.path.join(d, "fill.txt")
     cg.filling_file(fill, FILL_BUCKETS, out=(0, 13))
-    text = cg.ring_conf(**VARIANTS[name]).replace("@FILL@", os.path.abspath(fill))
+    v = dict(VARIANTS[name])
+    secs = {k: dict(x) for k, x in v.get("sections", {}).items()}
+    for sec, keys in section_overrides.items():
+        secs.setdefault(sec, {}).update(keys)
+    v["sections"] = secs
+    text = cg.ring_conf(**v).replace("@FILL@", os.path.abspath(fill))
     return cg.write(os.path.join(d, name + ".conf"), text)
