@@ -107,6 +107,9 @@ def batched(steps, warmup, nrings, seed):
         R, G, N = ens.size()
         if warmup > 0:
             ens.track(warmup)
+        g0 = time.perf_counter()
+        ens.track(steps)
+        g1 = time.perf_counter()
         ens.set_timing(True)
         t0 = time.perf_counter()
         ens.track(steps)
@@ -127,7 +130,9 @@ def batched(steps, warmup, nrings, seed):
             "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                    "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
+                               "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)}}
 
 
 def copy_calibration():
@@ -173,6 +178,11 @@ def main():
         nb, npb = ring.Nb, ring.Np
         if a.warmup > 0:
             ring.track(a.warmup)
+        # the same number of turns replayed as HIP graphs without timing events
+        g0 = time.perf_counter()
+        ring.track(a.steps)
+        g1 = time.perf_counter()
+        # timed region: every kernel bracketed by HIP events on the ring's stream
         ring.set_timing(True)
         t0 = time.perf_counter()
         ring.track(a.steps)
@@ -204,6 +214,9 @@ def main():
                                "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+        "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
+                           "graphs": not a.no_graphs, "ms_per_step": 1e3 * (g1 - g0) / a.steps,
+                           "value": nb * npb * a.steps / (g1 - g0)},
     }
     if not a.no_batched:
         try:

@@ -44,6 +44,16 @@
 #define TURN_WAVES_PER_EU 4
 #endif
 #define STATS_TURNS 256
+#ifndef COORD_NONTEMPORAL
+#define COORD_NONTEMPORAL 1
+#endif
+#if COORD_NONTEMPORAL
+#define LDC(p) __builtin_nontemporal_load(p)
+#define STC(v, p) __builtin_nontemporal_store((v), (p))
+#else
+#define LDC(p) (*(p))
+#define STC(v, p) (*(p) = (v))
+#endif
 #define MAXLR (3 * MBT_MAX_LR)
 
 #define CK(x)                                                                                    \
@@ -449,7 +459,9 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
   const unsigned jloc0 = (unsigned)(j0 - bunch_first[b]);
 
   for (unsigned i = tid; i < cnt; i += WG) {
-    double tau = pt[i], del = pd[i], x = px[i], xp = pxp[i], z = pz[i], zp = pzp[i];
+    // coordinates stream through once per turn: nontemporal loads and stores
+    double tau = LDC(pt + i), del = LDC(pd + i), x = LDC(px + i), xp = LDC(pxp + i), z = LDC(pz + i),
+           zp = LDC(pzp + i);
     if (rw_h) xp += rwx;
     if (rw_v) zp += rwz;
     if (MODE == MODE_TURN) {
@@ -521,12 +533,12 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
       }
     }
     if (MODE != MODE_STATS) {
-      pt[i] = tau;
-      pd[i] = del;
-      px[i] = x;
-      pxp[i] = xp;
-      pz[i] = z;
-      pzp[i] = zp;
+      STC(tau, pt + i);
+      STC(del, pd + i);
+      STC(x, px + i);
+      STC(xp, pxp + i);
+      STC(z, pz + i);
+      STC(zp, pzp + i);
     }
     // ---- moments about the per-bunch shift (reassociated: no bitwise target)
     const double v[6] = {tau, del, x, xp, z, zp};
@@ -1129,7 +1141,6 @@ struct GraphEntry {
   hipGraphExec_t exec;
   long long phase; // first turn modulo NrevMon
   long n;          // turns
-  int timed;       // records the timing events
 };
 
 struct mbt_ring {
@@ -1881,6 +1892,7 @@ static int download_slots(mbt_ring *r, long long rev0, long n, double *stats, do
 extern "C" int mbt_ring_prepare(mbt_ring *r, double *init_stats)
 {
   CK(hipSetDevice(r->device));
+  (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   int rc;
   // two stats passes: means (shift 0), then moments about the means + histograms
   CK(hipMemsetAsync(r->shift, 0, (size_t)r->G * 6 * sizeof(double), r->stream));
@@ -1948,13 +1960,13 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
 // n turns from the current turn as one HIP graph.  The launches of a turn
 // depend on the turn number only through the monitor-turn variant of k_turn,
 // so a graph is reusable for any block of n turns at the same phase modulo
-// NrevMon; a few are cached.  With timed, the graph also records the timing
-// events of every turn (event record nodes).
-static int turn_graph(mbt_ring *r, long n, int timed, hipGraphExec_t *out)
+// NrevMon; a few are cached.  (Timed runs launch eagerly: the runtime does not
+// time events recorded inside a graph.)
+static int turn_graph(mbt_ring *r, long n, hipGraphExec_t *out)
 {
   const long long phase = r->rev_host % r->nrev_mon;
   for (int i = 0; i < NGRAPH; i++)
-    if (r->graphs[i].exec && r->graphs[i].phase == phase && r->graphs[i].n == n && r->graphs[i].timed == timed) {
+    if (r->graphs[i].exec && r->graphs[i].phase == phase && r->graphs[i].n == n) {
       *out = r->graphs[i].exec;
       return MBT_OK;
     }
@@ -1967,7 +1979,7 @@ static int turn_graph(mbt_ring *r, long n, int timed, hipGraphExec_t *out)
   const long long rev_save = r->rev_host;
   CK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
   int rc = MBT_OK;
-  for (long t = 0; t < n && rc == MBT_OK; t++) rc = enqueue_turn(r, timed ? r->evpool + 6 * t : NULL);
+  for (long t = 0; t < n && rc == MBT_OK; t++) rc = enqueue_turn(r, NULL);
   r->rev_host = rev_save;
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(r->stream, &graph);
@@ -1988,7 +2000,6 @@ static int turn_graph(mbt_ring *r, long n, int timed, hipGraphExec_t *out)
   }
   g.phase = phase;
   g.n = n;
-  g.timed = timed;
   *out = g.exec;
   return MBT_OK;
 }
@@ -1996,6 +2007,7 @@ static int turn_graph(mbt_ring *r, long n, int timed, hipGraphExec_t *out)
 extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *ampinv, double *rf)
 {
   CK(hipSetDevice(r->device));
+  (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   int rc;
   for (int i = 0; i < 4; i++) {
     r->kt_ms[i] = 0;
@@ -2011,12 +2023,12 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
     if (n > block) n = block;
     long long rev0 = r->rev_host;
     const bool timed = r->timing != 0;
-    if (!r->use_graphs) {
+    if (timed || !r->use_graphs) {
       for (long t = 0; t < n; t++)
         if ((rc = enqueue_turn(r, timed ? r->evpool + 6 * t : NULL))) return rc;
     } else {
       hipGraphExec_t exec;
-      if ((rc = turn_graph(r, n, timed, &exec))) return rc;
+      if ((rc = turn_graph(r, n, &exec))) return rc;
       CK(hipGraphLaunch(exec, r->stream));
       r->rev_host += n;
     }
@@ -2048,6 +2060,7 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
 extern "C" int mbt_ring_final_stats(mbt_ring *r, double *stats)
 {
   CK(hipSetDevice(r->device));
+  (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   int rc;
   // apply the pending RW kick, then two exact passes for the statistics
   if ((rc = launch_turn_pass(r, MODE_FINAL))) return rc;

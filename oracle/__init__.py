@@ -52,6 +52,12 @@ def lib():
         L.orc_sincos2pif.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         if hasattr(L, "orc_wofz"):
             L.orc_wofz.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+        L.orc_ws_create.argtypes = [vp, ip, dp]
+        L.orc_ws_create.restype = vp
+        L.orc_ws_destroy.argtypes = [vp]
+        L.orc_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
+        L.orc_ws_get_slices.argtypes = [vp, ip, dp]
+        L.orc_ws_population.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -111,6 +117,60 @@ class OracleRing:
         t = (ctypes.c_int * (2 * self.Nb))()
         lib().orc_ring_histograms(self.ptr, _dp(c), t)
         return c, np.array(list(t)).reshape(self.Nb, 2)
+
+
+WS_IBUNCH_REFERENCE = 0
+WS_IBUNCH_FILL = 1
+
+
+class OracleWS:
+    """CPU restatement of the weak-strong turn with fast beam-ion interaction
+    (oracle_ws.c); the same call shape as mtrack_amd.WeakStrong."""
+
+    def __init__(self, cfg, ibunch_mode=WS_IBUNCH_FILL):
+        self.cfg = cfg
+        self.Nb = cfg.geti("Nbunch")
+        self.h = cfg.geti("Nharm")
+        self.S = 2 * cfg.geti("Nslc") + 1
+        self.Np = cfg.geti("Np")
+        self.dist = np.zeros(6 * self.Np)
+        p = lib().orc_ws_create(cfg.ptr, ibunch_mode, _dp(self.dist))
+        if not p:
+            raise RuntimeError("oracle cannot represent this weak-strong configuration")
+        self.ptr = ctypes.c_void_p(p)
+        self.dist = self.dist.reshape(6, self.Np)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().orc_ws_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def track(self, nturns):
+        cm = np.zeros((nturns, self.Nb, 4))
+        mon = np.zeros((nturns, self.Nb, 18))
+        ion = np.zeros((nturns, self.Nb, 3))
+        zcm = np.zeros((nturns, self.h))
+        rc = lib().orc_ws_track(self.ptr, nturns, _dp(cm), _dp(mon), _dp(ion), _dp(zcm))
+        return cm, mon, ion, zcm, rc
+
+    def slices(self, ib):
+        out = np.zeros(6 * self.S)
+        lib().orc_ws_get_slices(self.ptr, ib, _dp(out))
+        return out.reshape(6, self.S)
+
+    def population(self):
+        npop = (ctypes.c_int * self.S)()
+        nofe = (ctypes.c_int * (25 * self.S))()
+        lib().orc_ws_population(self.ptr, npop, nofe)
+        return np.array(list(npop)), np.array(list(nofe)).reshape(self.S, 5, 5)
+
+
+def wofz(x, y):
+    u, v = ctypes.c_double(), ctypes.c_double()
+    f = lib().orc_wofz(float(x), float(y), ctypes.byref(u), ctypes.byref(v))
+    return u.value, v.value, f
 
 
 def generate_bunch(cfg, kb):

@@ -121,6 +121,10 @@ static double orc_c_fnorm(rank_rng *s, int iseed)
   return sum;
 }
 
+/* one rank's c_fnorm stream for the other oracle units (oracle_ws.c) */
+void *orc_rank_rng_new(void) { return calloc(1, sizeof(rank_rng)); }
+double orc_rank_c_fnorm(void *s, int iseed) { return orc_c_fnorm((rank_rng *)s, iseed); }
+
 /* box_muller, src/bunch.c:407-436: Marsaglia polar method in float */
 static double orc_box_muller(rank_rng *s, int iseed)
 {

@@ -41,4 +41,15 @@ void orc_sincos2pif(float t, float *s, float *c);
 /* Faddeeva w(z), ACM TOMS 680 restated (src/physic.c:75-301); returns 0 or 1 (overflow) */
 int orc_wofz(double x, double y, double *u, double *v);
 
+/* ---- weak-strong turn with fast beam-ion interaction (oracle_ws.c) */
+#define ORC_WS_IBUNCH_REFERENCE 0 /* ring.Ibunch left zero, as the reference's STRONG mode does */
+#define ORC_WS_IBUNCH_FILL 1      /* bunch currents of the filling (the weak-weak manager's rule) */
+typedef struct orc_ws orc_ws;
+/* dist (may be NULL) receives the generated macro-electrons, 6 x Np (tau, delta, x, x', z, z') */
+orc_ws *orc_ws_create(const mbt_config *c, int ibunch_mode, double *dist);
+void orc_ws_destroy(orc_ws *w);
+int orc_ws_track(orc_ws *w, long nturns, double *cm, double *mon, double *ion, double *zcm);
+int orc_ws_get_slices(const orc_ws *w, int ib, double *out);
+int orc_ws_population(const orc_ws *w, int *npop, int *nofe);
+
 #endif

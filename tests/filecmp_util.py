@@ -14,8 +14,10 @@ def last_digit_unit(tok):
     return 10.0 ** (exp - len(frac))
 
 
-def compare(a_text, b_text, units=2.0):
-    """Returns a list of mismatch descriptions (empty when equivalent)."""
+def compare(a_text, b_text, units=2.0, a_nan_matches_number=False):
+    """Returns a list of mismatch descriptions (empty when equivalent).  With
+    a_nan_matches_number, a nan token of the first text (the reference's
+    output) stands for any number of the second."""
     out = []
     al, bl = a_text.splitlines(), b_text.splitlines()
     if len(al) != len(bl):
@@ -27,6 +29,8 @@ def compare(a_text, b_text, units=2.0):
             continue
         for p, q in zip(xt, yt):
             if p == q:
+                continue
+            if a_nan_matches_number and p.lstrip("+-").lower() == "nan" and NUM.match(q):
                 continue
             if NUM.match(p) and NUM.match(q):
                 tol = units * max(last_digit_unit(p), last_digit_unit(q))

@@ -69,6 +69,10 @@ def make(name):
 
 
 
+# per-variant section overrides of the end-to-end cases (none at present)
+E2E_OVERRIDES = {}
+E2E_CASES = ["wakes", "lr_fb", "rsis"]
+
 MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
@@ -80,7 +84,7 @@ def make_e2e(name="wakes"):
     import shutil
     out = os.path.join(HERE, "e2e_" + name)
     with tempfile.TemporaryDirectory() as d:
-        conf = write_variant(name, d)
+        conf = write_variant(name, d, **E2E_OVERRIDES.get(name, {}))
         work = os.path.join(d, "work")
         os.makedirs(work)
         st = np.loadtxt  # noqa: F841
@@ -109,11 +113,49 @@ def make_e2e(name="wakes"):
     return out
 
 
+WS_HARNESS = os.path.join(ROOT, "oracle", "_ref", "refwsharness")
+WOFZ_DUMP = os.path.join(ROOT, "oracle", "_ref", "refwofz")
+
+
+def make_ws(name):
+    """Weak-strong fixture from the reference's own functions
+    (oracle/_ref/refwsharness): per-turn CMs, monitor statistics, ion-cloud sums,
+    bucket z-centroids, final slices, and the first 64 generated macro-electrons."""
+    from variants import write_ws_variant
+    with tempfile.TemporaryDirectory() as d:
+        conf, mode = write_ws_variant(name, d)
+        text = open(conf).read().replace(os.path.abspath(os.path.join(d, "fill_ws.txt")), "@FILL@")
+        r = subprocess.run([WS_HARNESS, conf, d, "0", str(mode)], capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        ld = lambda f: np.loadtxt(os.path.join(d, f), ndmin=2)  # noqa: E731
+        dist = np.concatenate([ld(f)[:64, 1:] for f in ("Ldist", "Hdist", "Vdist")], axis=1)
+        out = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(out, conf=np.array(text), mode=mode, cm=ld("cm.txt"), mon=ld("mon.txt"),
+                            ion=ld("ion.txt"), zcm=ld("zcm.txt"), slices=ld("slices.txt"), dist=dist)
+    return out
+
+
+def make_wofz():
+    """w(z) of the reference's complex_errorfunc_wofz_algo680 on 1,082 points."""
+    r = subprocess.run([WOFZ_DUMP], capture_output=True, text=True, check=True)
+    a = np.array([[float(t) for t in l.split()] for l in r.stdout.splitlines()])
+    out = os.path.join(HERE, "wofz_ref.npz")
+    np.savez_compressed(out, x=a[:, 0], y=a[:, 1], u=a[:, 2], v=a[:, 3], flag=a[:, 4].astype(int))
+    return out
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or sorted(VARIANTS) + ["e2e:wakes"]
+    from variants import WS_VARIANTS
+    names = sys.argv[1:] or (sorted(VARIANTS) + ["e2e:" + c for c in E2E_CASES] + sorted(WS_VARIANTS)
+                             + ["wofz"])
     for n in names:
         if n.startswith("e2e:"):
             print(n, make_e2e(n[4:]))
+        elif n in WS_VARIANTS:
+            print(n, make_ws(n))
+        elif n == "wofz":
+            print(n, make_wofz())
         else:
             p = make(n)
             print(n, os.path.getsize(p))

@@ -1,10 +1,32 @@
 """The command-line drop-in (mbtrack-mi355x, host C driver over the GPU ring)
 against the unmodified reference program's output files for the same input
-(tests/golden/e2e_wakes, written by mbtrack-mpi under MPICH with 21 ranks)."""
+(tests/golden/e2e_<case>, written by mbtrack-mpi under MPICH, one rank per
+bunch plus the manager).
+
+Cases: "lr_fb" (harmonic cavities as long-range resonators with RF feedback)
+and "rsis" (broadband resonator with resistive and inductive smoothing) are
+compared file by file, token by token.  "wakes" adds the resistive-wall
+long-range kick: the reference reads beffH3/beffV3 only when EnableRW_short is
+set (src/input.c:465-478) and so divides by zero in every HOR/VER quantity
+(its files hold nan there), while this build reads them whenever the kick needs
+them; for that case a reference nan must face a number here and every other
+token must match.
+
+Two known departures are masked, each where it arises:
+  * potentials_bunch_<kb>.dat, bunch_shape of slice 0: the reference's
+    convolution loop stores one element past GL1 (`icell <= Ncell`,
+    src/transform_weak.c:658-667), which on its stack lands on fnp[0], so it
+    prints that store instead of the slice count;
+  * final distributions (Ldist/Hdist/Vdist): every coordinate is printed to 12
+    digits, also those that happen to lie near zero, where the GPU's 1e-15
+    relative agreement with the reference shows in the printed digits; they are
+    compared per column to 1e-10 of the column's largest magnitude.
+"""
 import os
-import shutil
+import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import mtrack_amd as m
@@ -12,34 +34,66 @@ from filecmp_util import compare
 from variants import write_variant
 
 pytestmark = pytest.mark.gpu
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "e2e_wakes")
+GOLD_ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def test_cli_files_match_reference(tmp_path):
+def mask_slice0_shape(text):
+    """Blanks the bunch_shape column (9th) of the first slice of every output block."""
+    out, prev_blank = [], False
+    for line in text.splitlines():
+        t = line.split()
+        if prev_blank and len(t) >= 9 and not line.lstrip().startswith("#"):
+            t[8] = "*"
+            line = " ".join(t)
+        prev_blank = line.strip() == ""
+        out.append(line)
+    return "\n".join(out)
+
+
+def compare_columns(a_text, b_text, rel):
+    """Numeric table comparison: |a - b| <= rel * max |a| of the column."""
+    a = np.array([[float(x) for x in l.split()] for l in a_text.splitlines() if l.strip()])
+    b = np.array([[float(x) for x in l.split()] for l in b_text.splitlines() if l.strip()])
+    if a.shape != b.shape:
+        return ["shape %s != %s" % (a.shape, b.shape)]
+    scale = np.maximum(np.abs(a).max(axis=0), 1e-300)
+    err = (np.abs(a - b) / scale).max(axis=0)
+    return ["column %d: %.3g > %g" % (c, e, rel) for c, e in enumerate(err) if e > rel]
+
+
+@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes"])
+def test_cli_files_match_reference(tmp_path, case):
+    gold = os.path.join(GOLD_ROOT, "e2e_" + case)
     d = str(tmp_path)
-    conf = write_variant("wakes", d)
+    conf = write_variant(case, d)
     work = os.path.join(d, "work")
     os.makedirs(work)
-    print("case e2e wakes", flush=True)
+    print("case e2e", case, flush=True)
     r = subprocess.run([m.EXE, conf, work + "/"], cwd=d, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    ref_files = sorted(f for f in os.listdir(GOLD) if f != "stdout.txt")
+    ref_files = sorted(f for f in os.listdir(gold) if f != "stdout.txt")
     produced = set(os.listdir(work)) | {f for f in ("Ldist", "Hdist", "Vdist") if os.path.exists(os.path.join(d, f))}
     missing = [f for f in ref_files if f not in produced]
     assert not missing, missing
     bad = {}
     for f in ref_files:
         mine = os.path.join(work, f) if os.path.exists(os.path.join(work, f)) else os.path.join(d, f)
-        errs = compare(open(os.path.join(GOLD, f)).read(), open(mine).read())
+        g_text, m_text = open(os.path.join(gold, f)).read(), open(mine).read()
+        if f in ("Ldist", "Hdist", "Vdist"):
+            errs = compare_columns(g_text, m_text, 1e-10)
+        else:
+            if f.startswith("potentials_bunch_"):
+                g_text, m_text = mask_slice0_shape(g_text), mask_slice0_shape(m_text)
+            errs = compare(g_text, m_text, a_nan_matches_number=(case == "wakes"))
         if errs:
             bad[f] = errs[:3]
     assert not bad, bad
     out = r.stdout.replace("\r", "")
-    import re
     out = re.sub(r"at time: [0-9:]+", "at time: T", out)
-    assert compare(open(os.path.join(GOLD, "stdout.txt")).read(), out) == []
+    assert compare(open(os.path.join(gold, "stdout.txt")).read(), out) == []
 
 
 def test_cli_usage_error():
-    r = subprocess.run([m.EXE], capture_output=True, text=True)
-    assert r.returncode == 255 and "Usage" in r.stderr
+    r = subprocess.run([m.EXE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255
+    assert "usage" in (r.stdout + r.stderr).lower()

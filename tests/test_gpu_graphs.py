@@ -38,8 +38,10 @@ def test_graph_replays_equal_eager_turns(tmp_path, name):
 
 
 def test_timed_graphs(tmp_path):
-    """Per-kernel timing events recorded inside the graph: same results, and
-    every kernel class of a turn with RW and LR resonators has a duration."""
+    """Timing mode (per-kernel HIP events; launched eagerly, since the runtime
+    does not time events recorded inside a graph) on a ring created with
+    graphs: same results as an eager ring, and every kernel class of a turn
+    with RW and LR resonators has a duration."""
     path = write_variant("lr_fb", str(tmp_path), tracking={"EnableRW_long": 1, "Nmlt": 2})
     cfg = m.Config(path, str(tmp_path) + "/")
     a = m.Ring(cfg, graphs=True)

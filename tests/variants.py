@@ -84,3 +84,38 @@ def write_variant(name, d, **section_overrides):
     v["sections"] = secs
     text = cg.ring_conf(**v).replace("@FILL@", os.path.abspath(fill))
     return cg.write(os.path.join(d, name + ".conf"), text)
+
+
+# ---- weak-strong (BunchModel = STRONG) configurations: slice-centroid bunches,
+# fast beam-ion interaction, resistive-wall long-range on the centroid history
+WS_FILL = {20: [int(k * 20 / 12) for k in range(12)], 416: [int(k * 416 / 12) for k in range(12)]}
+
+
+def _ws(h, fbii, rw, nmlt=3, turns=30, np_=2000):
+    return dict(sections={
+        "tracking": {"BunchModel": "STRONG", "EnableFBII": fbii, "NrevTot": turns, "NrevMon": 10,
+                     "EnableRW_long": rw, "Nmlt": nmlt, "EnableQuantum": 0, "filling": "@FILL@"},
+        "ring": dict(Lc=cg.small_ring(h), Iring=100.0, **RW),
+        "distribution": {"Np": np_, "Nslc": 20}})
+
+
+# name: (configuration, bunch-current mode: 1 = currents of the filling, 0 = the
+# reference's STRONG-mode zeros)
+WS_VARIANTS = {
+    "ws_fbii": (_ws(20, 1, 1), 1),
+    "ws_fbii_refcur": (_ws(20, 1, 1), 0),
+    "ws_optics": (_ws(20, 0, 0), 1),
+    "ws_h416": (_ws(416, 1, 1, nmlt=12), 1),
+}
+
+
+def write_ws_variant(name, d):
+    """Writes a weak-strong variant's .conf and filling file into d; returns
+    (path, bunch-current mode)."""
+    v, mode = WS_VARIANTS[name]
+    os.makedirs(d, exist_ok=True)
+    h = round(v["sections"]["ring"]["Lc"] * 352.3e6 / 2.997925e8)
+    fill = os.path.join(d, "fill_ws.txt")
+    cg.filling_file(fill, WS_FILL[h], out=())
+    text = cg.ring_conf(**v).replace("@FILL@", os.path.abspath(fill))
+    return cg.write(os.path.join(d, name + ".conf"), text), mode
