@@ -243,6 +243,43 @@ int mbt_ensemble_get_bunch(const mbt_ring *r, int g, double *soa6);
 /* Number of rings, total bunches and total macroparticles. */
 int mbt_ensemble_size(const mbt_ring *r, int *nrings, int *nbunch_total, long long *nparticles);
 
+/* ---- weak-strong turn with fast beam-ion interaction (BunchModel = STRONG) --- */
+/* Slice-centroid bunches (2 Nslc + 1 slices each) with the FBII ion train and the
+ * resistive-wall kick on the centroid history.  Replaces manager_weakstrong /
+ * worker_weakstrong (src/manager_weakstrong.c:7-263, src/worker_weakstrong.c:14-247):
+ * strong_generate_eCMs + generate_initial_general_distribution2
+ * (src/bunch_strong.c:10-216), transformSbunch_shortrange / _optic / _longrange
+ * (src/rw_weakstrong.c:10-253), fbi_weakstrong_bunch / _train
+ * (src/fbi_weakstrong.c:49-451), bunch_strong_distribution_statistics
+ * (src/bunch.c:516-550). */
+typedef struct mbt_ws mbt_ws;
+/* flags: bunch currents of the filling (the weak-weak manager's rule,
+ * src/manager_weakweak.c:58-80) instead of the zeros the reference leaves in
+ * ring.Ibunch in this mode (which make the ions chargeless) */
+#define MBT_WS_BUNCH_CURRENTS 1
+int mbt_ws_create(const mbt_config *cfg, int device, int flags, mbt_ws **out);
+void mbt_ws_destroy(mbt_ws *w);
+/* nturns turns; per turn (any output may be NULL):
+ *   cm[t][ib][4]    x, x', z, z' centroid each bunch reports after its synchrotron step
+ *   mon[t][ib][18]  on monitor turns (turn % NrevMon == 0): pos[3] pos_sigma[3]
+ *                   slope[3] slope_sigma[3] ampinv[3] ampinv_cm[3], else 0
+ *   ion[t][ib][3]   ion cloud after bunch ib's passage: sum Nri x, sum Nri y, sum Nri
+ *   zcm[t][kb]      slice-averaged z of every bucket at the end of the turn */
+int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, double *ion, double *zcm);
+/* slice arrays of filled bunch ordinal ib: out[6][S] = tau, delta, x, x', z, z' */
+int mbt_ws_get_slices(const mbt_ws *w, int ib, double *out);
+/* macro-electrons per slice npop[S]; *nslices = S */
+int mbt_ws_population(const mbt_ws *w, int *npop, int *nslices);
+/* the generated macro-electrons dist[6][Np] */
+int mbt_ws_distribution(const mbt_ws *w, double *dist);
+long mbt_ws_turn(const mbt_ws *w);
+/* host set-up pieces (no device): FBII local grid spacing; slice centroids of
+ * every bucket slices[h][6][S], npop[S], grid counts nofe[S][5][5], dist[6][Np] */
+void mbt_ws_grid(const mbt_config *cfg, double *agLOCx, double *agLOCz);
+int mbt_ws_generate(const mbt_config *cfg, double *slices, int *npop, int *nofe, double *dist);
+/* manager_weakstrong + its workers on one GPU: the reference's files in cfg->work_path */
+int mbt_manager_weakstrong(const mbt_config *cfg, int device, int flags);
+
 /* ---- end-to-end drop-in ----------------------------------------------------- */
 /* Runs manager_weakweak + all workers of the reference on one GPU and writes the
  * reference's output files into cfg->work_path. */

@@ -18,6 +18,7 @@ NSTAT = 12
 NAMP = 6
 MBT_RING_NO_INIT_DIST = 1
 MBT_RING_NO_GRAPHS = 2
+MBT_WS_BUNCH_CURRENTS = 1
 
 _lib = None
 
@@ -62,6 +63,14 @@ def lib():
         L.mbt_ensemble_set_bunch.argtypes = [vp, ip, dp]
         L.mbt_ensemble_get_bunch.argtypes = [vp, ip, dp]
         L.mbt_ensemble_size.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_longlong)]
+        L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
+        L.mbt_ws_destroy.argtypes = [vp]
+        L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
+        L.mbt_ws_get_slices.argtypes = [vp, ip, dp]
+        L.mbt_ws_population.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+        L.mbt_ws_distribution.argtypes = [vp, dp]
+        L.mbt_ws_generate.argtypes = [vp, dp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp]
+        L.mbt_manager_weakstrong.argtypes = [vp, ip, ip]
         if hasattr(L, "mbt_manager_weakweak"):
             L.mbt_manager_weakweak.argtypes = [vp, ip]
         if hasattr(L, "mbt_wofz"):
@@ -241,3 +250,51 @@ class Ensemble(Ring):
         rf = np.zeros((nturns, self.R, 2)) if want_rf else None
         _check(lib().mbt_ring_track(self.ptr, nturns, dptr(st), dptr(am), dptr(rf)))
         return st, am, rf
+
+
+def ws_generate(cfg):
+    """Host set-up of the weak-strong model (no device): slice centroids of every
+    bucket [h][6][S], npop[S], grid counts [S][5][5], macro-electrons [6][Np]."""
+    h, S, Np = cfg.geti("Nharm"), 2 * cfg.geti("Nslc") + 1, cfg.geti("Np")
+    sl = np.zeros(h * 6 * S)
+    npop = (ctypes.c_int * S)()
+    nofe = (ctypes.c_int * (25 * S))()
+    dist = np.zeros(6 * Np)
+    _check(lib().mbt_ws_generate(cfg.ptr, dptr(sl), npop, nofe, dptr(dist)))
+    return (sl.reshape(h, 6, S), np.array(list(npop)), np.array(list(nofe)).reshape(S, 5, 5),
+            dist.reshape(6, Np))
+
+
+class WeakStrong:
+    """Weak-strong bunches with the fast beam-ion interaction on one GPU
+    (mbt_ws_* C ABI)."""
+
+    def __init__(self, cfg, device=0, bunch_currents=True):
+        L = lib()
+        self.cfg = cfg
+        self.Nb = cfg.geti("Nbunch")
+        self.h = cfg.geti("Nharm")
+        self.S = 2 * cfg.geti("Nslc") + 1
+        p = ctypes.c_void_p()
+        _check(L.mbt_ws_create(cfg.ptr, device, MBT_WS_BUNCH_CURRENTS if bunch_currents else 0, ctypes.byref(p)))
+        self.ptr = p
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().mbt_ws_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def track(self, nturns):
+        cm = np.zeros((nturns, self.Nb, 4))
+        mon = np.zeros((nturns, self.Nb, 18))
+        ion = np.zeros((nturns, self.Nb, 3))
+        zcm = np.zeros((nturns, self.h))
+        _check(lib().mbt_ws_track(self.ptr, nturns, dptr(cm), dptr(mon), dptr(ion), dptr(zcm)))
+        return cm, mon, ion, zcm
+
+    def slices(self, ib):
+        out = np.zeros(6 * self.S)
+        _check(lib().mbt_ws_get_slices(self.ptr, ib, dptr(out)))
+        return out.reshape(6, self.S)

@@ -18,8 +18,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmbtrack_mi355x.so")
 EXE = os.path.join(HERE, "mbtrack-mi355x")
 
-HOST_C = ["conf.c", "bunchgen.c", "writers.c", "driver.c"]
-HIP_SRC = ["ring.hip", "wofz.hip"]
+HOST_C = ["conf.c", "bunchgen.c", "writers.c", "driver.c", "ws_driver.c"]
+HIP_SRC = ["ring.hip", "wofz.hip", "ws.hip"]
 ARCH = "gfx950"
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",

@@ -350,3 +350,132 @@ int mbt_generate_all(const mbt_config *c, double *out)
   for (int i = 0; i < started; i++) pthread_join(th[i], NULL);
   return j.rc;
 }
+
+/* ------------------------------------------------- weak-strong (STRONG) set-up */
+
+/* FBII grid of fbii_grid_setup_parameters, src/bunch.c:814-850: Ngrid 5 points
+ * per axis (offset 2), local spacing sigma/3 [m], absolute spacing twice that. */
+void mbt_ws_grid(const mbt_config *c, double *agLOCx, double *agLOCz)
+{
+  *agLOCz = c->sgm_zz / ((double)3) / MBT_FKILO;
+  *agLOCx = c->sgm_xx / ((double)3) / MBT_FKILO;
+}
+
+/* Slice-centroid bunches of the STRONG model, as the reference's manager rank
+ * makes them: strong_generate_eCMs (src/bunch_strong.c:60-216) draws Np
+ * macro-electrons per plane from one c_fnorm stream (seeds iseed[plane]),
+ * sorts them into 2 Nslc + 1 longitudinal cells of 0.3 sigma from -7 sigma
+ * (cell index clamped to [0, 2 Nslc + 1]), averages x, x', z, z' per cell and
+ * counts them on each cell's 5 x 5 local grid; then
+ * generate_initial_general_distribution2 (src/bunch_strong.c:10-50) offsets
+ * every filled bucket's centroids by four more draws of the same stream
+ * (seed 571030).  The cells' tau and delta stay zero: the reference never
+ * assigns them.
+ *   slices[kb][6][S]  xtau xeps xx xp zz zp of bucket kb (filled buckets only)
+ *   npop[S], nofe[S][5][5], dist[6][Np] (may be NULL)                          */
+int mbt_ws_generate(const mbt_config *c, double *slices, int *npop, int *nofe, double *dist)
+{
+  const int Np = c->Np, S = 2 * c->Nslc + 1, H = (int)c->Nharm;
+  if (Np <= 0 || c->Nslc < 0 || S > 50) {
+    mbt_set_error("weak-strong slices: need Np > 0 and 2 Nslc + 1 <= 50 (Nslc %d)", c->Nslc);
+    return MBT_EINVAL;
+  }
+  double *m = malloc(sizeof(double) * 6 * (size_t)Np);
+  if (!m) return MBT_ENOMEM;
+  double *tau = m, *del = m + Np, *x = m + 2 * (size_t)Np, *xp = m + 3 * (size_t)Np;
+  double *z = m + 4 * (size_t)Np, *zp = m + 5 * (size_t)Np;
+  gen_state s;
+  memset(&s, 0, sizeof s);
+  int rc = MBT_OK;
+  if (Np == 1) {
+    tau[0] = c->xtauCM_offset * MBT_FNANO;
+    del[0] = c->xepsCM_offset;
+    x[0] = c->xxCM_offset * MBT_FMILLI;
+    xp[0] = c->xpCM_offset * MBT_FMILLI;
+    z[0] = c->zzCM_offset * MBT_FMILLI;
+    zp[0] = c->zpCM_offset * MBT_FMILLI;
+  } else {
+    if (c->nGen[0])
+      for (int j = 0; j < Np; j++) {
+        tau[j] = (c->xtauCM_offset + c->sgm_xtau * gen_fnorm(&s, c->iseed[0])) * MBT_FNANO;
+        del[j] = c->xepsCM_offset + c->sgm_xeps * gen_fnorm(&s, c->iseed[0]);
+      }
+    else rc = read_plane_file(c->gendst_datafile[0], Np, MBT_FNANO, 1.0, tau, del);
+    if (rc == MBT_OK) {
+      if (c->nGen[1])
+        for (int j = 0; j < Np; j++) {
+          double a = c->sgm_xx * gen_fnorm(&s, c->iseed[1]), b = c->sgm_xp * gen_fnorm(&s, c->iseed[1]);
+          x[j] = a * MBT_FMILLI;
+          xp[j] = b * MBT_FMILLI;
+        }
+      else rc = read_plane_file(c->gendst_datafile[1], Np, MBT_FMILLI, MBT_FMILLI, x, xp);
+    }
+    if (rc == MBT_OK) {
+      if (c->nGen[2])
+        for (int j = 0; j < Np; j++) {
+          double a = c->sgm_zz * gen_fnorm(&s, c->iseed[2]), b = c->sgm_zp * gen_fnorm(&s, c->iseed[2]);
+          z[j] = a * MBT_FMILLI;
+          zp[j] = b * MBT_FMILLI;
+        }
+      else rc = read_plane_file(c->gendst_datafile[2], Np, MBT_FMILLI, MBT_FMILLI, z, zp);
+    }
+  }
+  if (rc != MBT_OK) {
+    free(m);
+    return rc;
+  }
+  if (dist) memcpy(dist, m, sizeof(double) * 6 * (size_t)Np);
+  double cx[51] = {0}, cxp[51] = {0}, cz[51] = {0}, czp[51] = {0};
+  int pop[52] = {0};
+  unsigned char *cell = malloc((size_t)Np);
+  for (int j = 0; j < Np; j++) {
+    double T = -tau[j] / c->sgmatau;
+    int ic = (int)((7.0 + T) / 0.3);
+    if (ic < 1) ic = 0;
+    if (ic > S) ic = S;
+    cell[j] = (unsigned char)ic;
+    pop[ic]++;
+    cx[ic] = cx[ic] + x[j];
+    cxp[ic] = cxp[ic] + xp[j];
+    cz[ic] = cz[ic] + z[j];
+    czp[ic] = czp[ic] + zp[j];
+  }
+  for (int ic = 0; ic < S; ic++)
+    if (pop[ic]) {
+      cx[ic] = cx[ic] / pop[ic];
+      cxp[ic] = cxp[ic] / pop[ic];
+      cz[ic] = cz[ic] / pop[ic];
+      czp[ic] = czp[ic] / pop[ic];
+    }
+  double aLx, aLz;
+  mbt_ws_grid(c, &aLx, &aLz);
+  memset(nofe, 0, sizeof(int) * 25 * (size_t)S);
+  for (int j = 0; j < Np; j++) {
+    const int ic = cell[j];
+    if (ic >= S) continue;
+    int nz = (int)(2.0 * (z[j] - cz[ic]) / aLz), mz = nz >= 0 ? (nz + 1) / 2 : (nz - 1) / 2;
+    int nx = (int)(2.0 * (x[j] - cx[ic]) / aLx), mx = nx >= 0 ? (nx + 1) / 2 : (nx - 1) / 2;
+    mz = mz > 2 ? 2 : (mz < -2 ? -2 : mz);
+    mx = mx > 2 ? 2 : (mx < -2 ? -2 : mx);
+    nofe[(ic * 5 + mx + 2) * 5 + mz + 2]++;
+  }
+  for (int ic = 0; ic < S; ic++) npop[ic] = pop[ic];
+  memset(slices, 0, sizeof(double) * 6 * (size_t)S * H);
+  for (int kb = 0; kb < H; kb++) {
+    if (!c->nfFill[kb]) continue;
+    double ox = c->xxCM_offset * MBT_FMILLI * gen_fnorm(&s, 571030);
+    double op = c->xpCM_offset * MBT_FMILLI * gen_fnorm(&s, 571030);
+    double oz = c->zzCM_offset * MBT_FMILLI * gen_fnorm(&s, 571030);
+    double oq = c->zpCM_offset * MBT_FMILLI * gen_fnorm(&s, 571030);
+    double *b = slices + (size_t)kb * 6 * S;
+    for (int ic = 0; ic < S; ic++) {
+      b[2 * S + ic] = cx[ic] + ox;
+      b[3 * S + ic] = cxp[ic] + op;
+      b[4 * S + ic] = cz[ic] + oz;
+      b[5 * S + ic] = czp[ic] + oq;
+    }
+  }
+  free(cell);
+  free(m);
+  return MBT_OK;
+}

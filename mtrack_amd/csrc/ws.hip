@@ -1,0 +1,835 @@
+// The weak-strong multi-bunch turn with fast beam-ion interaction (FBII) on one
+// MI355X: the reference's BunchModel = STRONG path, manager_weakstrong +
+// worker_weakstrong (src/manager_weakstrong.c:7-263, src/worker_weakstrong.c:14-247).
+//
+// Each filled bunch is 2 Nslc + 1 slice centroids (tau, delta, x, x', z, z')
+// resident in HBM.  One turn is four launch groups on one stream:
+//
+//   k_ws_phase1  one block per bunch: synchrotron step of the centroids
+//                (transformSbunch_shortrange, src/rw_weakstrong.c:10-67), bunch
+//                centroid, CM history append (src/bunch.c:94-115), monitor-turn
+//                statistics (src/bunch.c:516-550); with FBII the Bassetti-Erskine
+//                field of every slice on the 5 x 5 absolute grid (two Faddeeva
+//                evaluations per point, wavefront-parallel) and the ions the
+//                bunch creates, kicked by its later slices
+//                (fbi_weakstrong_bunch, src/fbi_weakstrong.c:49-298);
+//   k_ws_phase2  one block per bunch: betatron rotation of the centroids
+//                (transformSbunch_optic, :194-253) and the resistive-wall kick of
+//                every other bucket's centroid history (transformSbunch_longrange,
+//                :69-192), one lane per source bucket;
+//   k_ws_train   bunch by bunch in train order (the reference's manager relay):
+//                the accumulated ion cloud drifts to the bunch, passes its slices
+//                (bilinear field interpolation, src/fbi_weakstrong.c:454-580),
+//                kicks the slice centroids with the summed ion field, and the
+//                bunch's own ions join the cloud (fbi_weakstrong_train :301-451);
+//                one lane per ion, per-slice block sums, the last block of each
+//                launch reduces them in a fixed order;
+//   k_ws_end     slice-averaged vertical centroid of every bucket.
+//
+// Everything but the ion-field sums, sin and the Faddeeva function keeps the
+// reference's operation order; constants derived with libm are computed on the
+// host with glibc.  Results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mbt_host.h"
+#include "wofz_dev.h"
+
+#pragma clang fp contract(off)
+
+#define WS_S 50       // slice capacity of the reference's bunch_strong_t
+#define WS_PTS 1250   // 50 slices x 25 grid points
+#define WS_HIST 30    // CM history depth (src/bunch.h:155)
+#define WS_MSG 5000   // doubles of the history message (src/manager_weakstrong.c:170-173)
+#define WS_T 256      // turns of outputs per device buffer block
+#define WS_WG 256
+
+#define K_C 2.997925e+08
+#define K_EPS0 8.854187817620e-12
+#define K_E 1.602e-19
+
+#define CK(x)                                                                                    \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) {                                                                      \
+      mbt_set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);     \
+      return MBT_EDEVICE;                                                                        \
+    }                                                                                            \
+  } while (0)
+
+extern "C" int mbt_ws_generate(const mbt_config *c, double *slices, int *npop, int *nofe, double *dist);
+extern "C" void mbt_ws_grid(const mbt_config *c, double *agLOCx, double *agLOCz);
+
+struct WSP {
+  int S, G, h, Nmlt, NmultiT, fbii, rw, track_h, track_v, track[3], Nions;
+  double eps, wrf, phai0, Urad, T0, ac;                 // synchrotron step
+  double gamma1[3], alpha1[3], beta1[3];                // statistics
+  double v11, v12, v21, v22, h11, h12, h21, h22;        // one-turn rotation (glibc cos/sin)
+  double rwa11[2], rwa12[2], prfrw[2], prfrw_trig[2];   // RW: [HOR, VER]
+  int triggerRW;
+  double Pref0, fNp, Dub0, Lc;
+  double factBE, sxx, szz, deltT, aAx, aAz, aLx, aLz, den_dQ; // FBII fields
+  double factF, factjp0, CdeltT;
+};
+
+struct WSD {
+  double *sl;       // [G][6][S] xtau xeps xx xp zz zp
+  const int *npop;  // [S]
+  const int *nofe;  // [S*25]
+  const int *bucket, *fill; // [G], [h]
+  const double *Ibunch, *nelec, *convion; // [h] mA, [G], [G]
+  double *hist;     // [4][h][WS_HIST] x x' z z' (the manager's history)
+  double *Ex, *Ey;  // [G][S*25]
+  double *li;       // [5][G][S*25] local ions: Nri x x' y y'
+  double *cl;       // [5][cap] ion cloud
+  long long cap;
+  double *part;     // [blocks][S][2]
+  double *psum;     // [blocks][3]
+  unsigned *done;
+  double *cm, *mon, *ion, *zcm; // output blocks [WS_T][G][4|18|3], [WS_T][h]
+};
+
+// ------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ double wsum64(double v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// bunch_strong_updateCM, src/bunch_strong.c:221-266 (sequential, one lane)
+__device__ void ws_cm(const double *u, const double *p, const int *npop, int S, double &pos, double &slope)
+{
+  double uu = 0.0, up = 0.0;
+  int n = 0;
+  for (int ic = 0; ic < S; ic++) {
+    uu += u[ic] * ((double)npop[ic]);
+    up += p[ic] * ((double)npop[ic]);
+    n += npop[ic];
+  }
+  pos = uu / ((double)n);
+  slope = up / ((double)n);
+}
+
+// get_linearly_interpolated_Efields, src/fbi_weakstrong.c:454-580, on one
+// slice's 25 field values
+__device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double xx,
+                                          double yy, double &Ex, double &Ey)
+{
+  int k1x = (int)(xx / aAx), k1z = (int)(yy / aAz), k2x, k2z;
+  const bool pz = yy >= 0, px = xx >= 0;
+  if (pz) {
+    k2z = k1z < 2 ? k1z + 1 : k1z;
+    k1z = k1z > 2 ? 2 : k1z;
+    k2z = k2z > 2 ? 2 : k2z;
+  } else {
+    k2z = k1z > -2 ? k1z - 1 : k1z;
+    k1z = k1z < -2 ? -2 : k1z;
+    k2z = k2z < -2 ? -2 : k2z;
+  }
+  if (px) {
+    k2x = k1x < 2 ? k1x + 1 : k1x;
+    k1x = k1x > 2 ? 2 : k1x;
+    k2x = k2x > 2 ? 2 : k2x;
+  } else {
+    k2x = k1x > -2 ? k1x - 1 : k1x;
+    k1x = k1x < -2 ? -2 : k1x;
+    k2x = k2x < -2 ? -2 : k2x;
+  }
+  const int i1 = (k1x + 2) * 5 + k1z + 2, i2 = (k2x + 2) * 5 + k1z + 2;
+  const int i3 = (k1x + 2) * 5 + k2z + 2, i4 = (k2x + 2) * 5 + k2z + 2;
+  const double fx = (xx - k1x * aAx) / aAx, fz = (yy - k1z * aAz) / aAz;
+  double y5, y6, x5, x6;
+  if (px) {
+    y5 = ey[i1] + fx * (ey[i2] - ey[i1]);
+    y6 = ey[i3] + fx * (ey[i4] - ey[i3]);
+    x5 = ex[i1] + fx * (ex[i2] - ex[i1]);
+    x6 = ex[i3] + fx * (ex[i4] - ex[i3]);
+  } else {
+    y5 = ey[i1] - fx * (ey[i2] - ey[i1]);
+    y6 = ey[i3] - fx * (ey[i4] - ey[i3]);
+    x5 = ex[i1] - fx * (ex[i2] - ex[i1]);
+    x6 = ex[i3] - fx * (ex[i4] - ex[i3]);
+  }
+  if (pz) {
+    Ey = y5 + fz * (y6 - y5);
+    Ex = x5 + fz * (x6 - x5);
+  } else {
+    Ey = y5 - fz * (y6 - y5);
+    Ex = x5 - fz * (x6 - x5);
+  }
+}
+
+// ------------------------------------------------------------------- phase 1
+
+__global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev, int slot, int monitor)
+{
+  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  const int b = blockIdx.x, tid = threadIdx.x, S = P.S;
+  const int kb = D.bucket[b];
+  double *xtau = D.sl + (size_t)b * 6 * S, *xeps = xtau + S, *xx = xeps + S, *xp = xx + S, *zz = xp + S,
+         *zp = zz + S;
+  for (int ic = tid; ic < S; ic += WS_WG) {
+    const double e0 = xeps[ic];
+    const double gain = P.eps * sin(P.wrf * xtau[ic] + P.phai0);
+    xeps[ic] = e0 + gain - P.Urad;
+    xtau[ic] = xtau[ic] - e0 * P.T0 * P.ac;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double pos[3], slope[3];
+    ws_cm(xtau, xeps, D.npop, S, pos[0], slope[0]);
+    ws_cm(xx, xp, D.npop, S, pos[1], slope[1]);
+    ws_cm(zz, zp, D.npop, S, pos[2], slope[2]);
+    double *o = D.cm + ((size_t)slot * P.G + b) * 4;
+    o[0] = pos[1];
+    o[1] = slope[1];
+    o[2] = pos[2];
+    o[3] = slope[2];
+    // the manager's history of this bucket: shift by one, newest at 0
+    const double nw[4] = {pos[1], slope[1], pos[2], slope[2]};
+    for (int a = 0; a < 4; a++) {
+      double *hrow = D.hist + ((size_t)a * P.h + kb) * WS_HIST;
+      for (int i = 0; i < P.NmultiT; i++) {
+        const int j = P.NmultiT - i - 2;
+        if (j >= 0) hrow[j + 1] = hrow[j];
+        else hrow[0] = nw[a];
+      }
+    }
+    if (monitor) {
+      double *m = D.mon + ((size_t)slot * P.G + b) * 18;
+      const double *U[3] = {xtau, xx, zz}, *Q[3] = {xeps, xp, zp};
+      for (int pl = 0; pl < 3; pl++) {
+        for (int k = 0; k < 6; k++) m[3 * k + pl] = 0.0;
+        if (!P.track[pl]) continue;
+        const double *u = U[pl], *p = Q[pl];
+        double avr, avs;
+        ws_cm(u, p, D.npop, S, avr, avs);
+        double sq = 0.0, sqs = 0.0, amp = 0.0;
+        int w1 = 0, w2 = 0, n = 0;
+        for (int ic = 0; ic < S; ic++) {
+          sq += (u[ic] - avr) * (u[ic] - avr) * D.npop[ic];
+          w1 += D.npop[ic];
+        }
+        for (int ic = 0; ic < S; ic++) {
+          sqs += (p[ic] - avs) * (p[ic] - avs) * D.npop[ic];
+          w2 += D.npop[ic];
+        }
+        for (int ic = 0; ic < S; ic++) {
+          amp += D.npop[ic] * (P.gamma1[pl] * (u[ic] * u[ic]) + 2.0 * P.alpha1[pl] * u[ic] * p[ic] +
+                               P.beta1[pl] * (p[ic] * p[ic]));
+          n += D.npop[ic];
+        }
+        m[pl] = avr;
+        m[3 + pl] = sqrt(sq / ((double)w1));
+        m[6 + pl] = avs;
+        m[9 + pl] = sqrt(sqs / ((double)w2));
+        m[12 + pl] = amp / ((double)n);
+        m[15 + pl] = P.gamma1[pl] * (avr * avr) + 2.0 * P.alpha1[pl] * avr * avs + P.beta1[pl] * (avs * avs);
+      }
+    }
+  }
+  if (!P.fbii) return;
+  // ---- Bassetti-Erskine field of every slice at the absolute grid points
+  const double nel = D.nelec[b];
+  const int npts = S * 25;
+  for (int q = tid; q < npts; q += WS_WG) {
+    const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
+    const double dQ = nel * D.npop[cell] * K_E;
+    const double fdQ = dQ / P.den_dQ;
+    const double Dx = msx * P.aAx - xx[cell];
+    const double xi1 = Dx / P.factBE, xi2 = xi1 * P.szz / P.sxx;
+    const double Dy = msz * P.aAz - zz[cell];
+    const double yi1 = Dy / P.factBE, yi2 = yi1 * P.sxx / P.szz;
+    double u1, v1, u2, v2;
+    mbt_wofz_dev<double>(xi1, yi1, u1, v1);
+    mbt_wofz_dev<double>(xi2, yi2, u2, v2);
+    const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
+    const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
+    ex_s[q] = Ex;
+    ey_s[q] = Ey;
+    D.Ex[(size_t)b * npts + q] = Ex;
+    D.Ey[(size_t)b * npts + q] = Ey;
+  }
+  __syncthreads();
+  // ---- the ions this bunch creates: one per local grid point, kicked by the
+  // bunch's later slices
+  const double conv = D.convion[b];
+  const size_t GP = (size_t)P.G * npts;
+  for (int q = tid; q < npts; q += WS_WG) {
+    const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
+    const double Nri = conv * ((double)D.nofe[q]);
+    double x = xx[cell] + msx * P.aLx, xd = 0.0, y = zz[cell] + msz * P.aLz, yd = 0.0;
+    for (int c = cell + 1; c < S; c++) {
+      double Ex, Ey;
+      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+      const double ax = P.factF * Ex;
+      xd = xd + P.deltT * ax;
+      x = x + P.deltT * xd;
+      const double ay = P.factF * Ey;
+      yd = yd + P.deltT * ay;
+      y = y + P.deltT * yd;
+    }
+    const size_t o = (size_t)b * npts + q;
+    D.li[o] = Nri;
+    D.li[GP + o] = x;
+    D.li[2 * GP + o] = xd;
+    D.li[3 * GP + o] = y;
+    D.li[4 * GP + o] = yd;
+  }
+}
+
+// ------------------------------------------------------------------- phase 2
+
+__global__ __launch_bounds__(WS_WG) void k_ws_phase2(WSP P, WSD D, long long rev)
+{
+  extern __shared__ double dup[]; // [2][h]
+  const int b = blockIdx.x, tid = threadIdx.x, S = P.S, h = P.h;
+  const int kb = D.bucket[b];
+  double *xx = D.sl + (size_t)b * 6 * S + 2 * S, *xp = xx + S, *zz = xp + S, *zp = zz + S;
+  for (int ic = tid; ic < S; ic += WS_WG) {
+    const double y = P.v11 * zz[ic] + P.v12 * zp[ic];
+    const double yp = P.v21 * zz[ic] + P.v22 * zp[ic];
+    zz[ic] = y;
+    zp[ic] = yp;
+    const double x = P.h11 * xx[ic] + P.h12 * xp[ic];
+    const double xq = P.h21 * xx[ic] + P.h22 * xp[ic];
+    xx[ic] = x;
+    xp[ic] = xq;
+  }
+  if (!P.rw) return;
+  // the kick of every filled source bucket jb, one lane each; the worker's copy
+  // of the history holds zeros where jb + j h falls outside the 5000-double message
+  for (int pl = 0; pl < 2; pl++) {
+    if (!(pl == 0 ? P.track_h : P.track_v)) continue;
+    const double *uh = D.hist + (size_t)(pl == 0 ? 0 : 2) * h * WS_HIST;
+    const double *ph = uh + (size_t)h * WS_HIST;
+    const double prf = (P.triggerRW && rev < 50) ? P.prfrw_trig[pl] : P.prfrw[pl];
+    for (int jb = tid; jb < h; jb += WS_WG) {
+      double Dup = 0.0;
+      if (D.fill[jb]) {
+        double D0;
+        if (jb == kb) D0 = P.Lc;
+        else if (jb > kb) D0 = ((double)(jb - kb)) * P.Dub0;
+        else D0 = ((double)(h + jb - kb)) * P.Dub0;
+        for (int j = 0; j < P.Nmlt + 1; j++) {
+          const double Du = D0 + ((double)j) * P.Lc;
+          const double Dt = Du / K_C;
+          const double W = prf / sqrt(Dt);
+          const int jm = jb > kb ? j : j + 1;
+          const bool in = jb + jm * h < WS_MSG;
+          const double u = in ? uh[(size_t)jb * WS_HIST + jm] : 0.0, p = in ? ph[(size_t)jb * WS_HIST + jm] : 0.0;
+          const double ujb = P.rwa11[pl] * u + P.rwa12[pl] * p;
+          const double dip = P.fNp * ujb;
+          Dup = Dup + dip * W;
+        }
+        Dup = P.Pref0 * D.Ibunch[jb] * Dup;
+        if (rev < P.Nmlt) Dup = 0.0;
+      }
+      dup[pl * h + jb] = Dup;
+    }
+  }
+  __syncthreads();
+  for (int pl = 0; pl < 2; pl++) {
+    if (!(pl == 0 ? P.track_h : P.track_v)) continue;
+    double *up = pl == 0 ? xp : zp;
+    for (int ic = tid; ic < S; ic += WS_WG) {
+      double a = up[ic];
+      for (int jb = 0; jb < h; jb++)
+        if (D.fill[jb]) a += dup[pl * h + jb];
+      up[ic] = a;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ the train
+
+// head bunch: the cloud becomes its ions (src/fbi_weakstrong.c:335-353)
+__global__ __launch_bounds__(WS_WG) void k_ws_cloud_reset(WSP P, WSD D, int slot)
+{
+  __shared__ double red[WS_WG / 64][3];
+  const int tid = threadIdx.x, npts = P.S * 25;
+  const size_t GP = (size_t)P.G * npts;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int q = tid; q < npts; q += WS_WG) {
+    for (int a = 0; a < 5; a++) D.cl[a * D.cap + q] = D.li[a * GP + q];
+    s0 += D.li[q] * D.li[GP + q];
+    s1 += D.li[q] * D.li[3 * GP + q];
+    s2 += D.li[q];
+  }
+  s0 = wsum64(s0);
+  s1 = wsum64(s1);
+  s2 = wsum64(s2);
+  if ((tid & 63) == 0) {
+    red[tid >> 6][0] = s0;
+    red[tid >> 6][1] = s1;
+    red[tid >> 6][2] = s2;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double s = 0.0;
+    for (int w = 0; w < WS_WG / 64; w++) s += red[w][tid];
+    D.ion[((size_t)slot * P.G) * 3 + tid] = s;
+  }
+}
+
+// bunch ib (bucket kb) meets the cloud of nF ions that drifted Dtime since the
+// previous bunch; the last block applies the slice kicks and appends the
+// bunch's ions
+__global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long long nF, double Dtime, int slot)
+{
+  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  __shared__ double red[WS_WG / 64][2];
+  __shared__ int last;
+  const int tid = threadIdx.x, S = P.S, npts = S * 25;
+  const size_t GP = (size_t)P.G * npts;
+  for (int q = tid; q < npts; q += WS_WG) {
+    ex_s[q] = D.Ex[(size_t)ib * npts + q];
+    ey_s[q] = D.Ey[(size_t)ib * npts + q];
+  }
+  __syncthreads();
+  const long long q = (long long)blockIdx.x * WS_WG + tid;
+  const bool act = q < nF;
+  double Nri = 0.0, x = 0.0, xd = 0.0, y = 0.0, yd = 0.0;
+  if (act) {
+    Nri = D.cl[q];
+    x = D.cl[D.cap + q];
+    xd = D.cl[2 * D.cap + q];
+    y = D.cl[3 * D.cap + q];
+    yd = D.cl[4 * D.cap + q];
+    x = x + Dtime * xd;
+    y = y + Dtime * yd;
+  }
+  double *part = D.part + (size_t)blockIdx.x * S * 2;
+  for (int c = 0; c < S; c++) {
+    double sx = 0.0, sy = 0.0;
+    if (act) {
+      double Ex, Ey;
+      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+      const double ax = P.factF * Ex;
+      xd = xd + P.deltT * ax;
+      x = x + P.deltT * xd;
+      const double ay = P.factF * Ey;
+      yd = yd + P.deltT * ay;
+      y = y + P.deltT * yd;
+      sx = Nri * Ex;
+      sy = Nri * Ey;
+    }
+    sx = wsum64(sx);
+    sy = wsum64(sy);
+    if ((tid & 63) == 0) {
+      red[tid >> 6][0] = sx;
+      red[tid >> 6][1] = sy;
+    }
+    __syncthreads();
+    if (tid < 2) {
+      double s = 0.0;
+      for (int w = 0; w < WS_WG / 64; w++) s += red[w][tid];
+      part[2 * c + tid] = s;
+    }
+    __syncthreads();
+  }
+  double s0 = act ? Nri * x : 0.0, s1 = act ? Nri * y : 0.0, s2 = act ? Nri : 0.0;
+  if (act) {
+    D.cl[D.cap + q] = x;
+    D.cl[2 * D.cap + q] = xd;
+    D.cl[3 * D.cap + q] = y;
+    D.cl[4 * D.cap + q] = yd;
+  }
+  s0 = wsum64(s0);
+  s1 = wsum64(s1);
+  s2 = wsum64(s2);
+  __shared__ double red3[WS_WG / 64][3];
+  if ((tid & 63) == 0) {
+    red3[tid >> 6][0] = s0;
+    red3[tid >> 6][1] = s1;
+    red3[tid >> 6][2] = s2;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double s = 0.0;
+    for (int w = 0; w < WS_WG / 64; w++) s += red3[w][tid];
+    D.psum[(size_t)blockIdx.x * 3 + tid] = s;
+  }
+  // ---- the last block to finish: fixed-order sums, slice kicks, append
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) last = atomicAdd(D.done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int kb = D.bucket[ib];
+  const double nel = D.nelec[ib];
+  double *xp = D.sl + (size_t)ib * 6 * S + 3 * S, *zp = xp + 2 * S;
+  for (int c = tid; c < S; c += WS_WG) {
+    double sx = 0.0, sy = 0.0;
+    for (unsigned k = 0; k < gridDim.x; k++) {
+      sx += D.part[((size_t)k * S + c) * 2];
+      sy += D.part[((size_t)k * S + c) * 2 + 1];
+    }
+    const double dQ = nel * D.npop[c] * K_E;
+    const double fjp = dQ > 0.0 ? P.factjp0 / dQ : 0.0;
+    sx = fjp * sx;
+    sy = fjp * sy;
+    xp[c] = xp[c] + P.CdeltT * sx;
+    zp[c] = zp[c] + P.CdeltT * sy;
+  }
+  // append the bunch's ions and finish the cloud sums
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  for (int p = tid; p < npts; p += WS_WG) {
+    const size_t o = (size_t)ib * npts + p;
+    for (int a = 0; a < 5; a++) D.cl[a * D.cap + nF + p] = D.li[a * GP + o];
+    t0 += D.li[o] * D.li[GP + o];
+    t1 += D.li[o] * D.li[3 * GP + o];
+    t2 += D.li[o];
+  }
+  t0 = wsum64(t0);
+  t1 = wsum64(t1);
+  t2 = wsum64(t2);
+  if ((tid & 63) == 0) {
+    red3[tid >> 6][0] = t0;
+    red3[tid >> 6][1] = t1;
+    red3[tid >> 6][2] = t2;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double s = 0.0;
+    for (unsigned k = 0; k < gridDim.x; k++) s += D.psum[(size_t)k * 3 + tid];
+    for (int w = 0; w < WS_WG / 64; w++) s += red3[w][tid];
+    D.ion[((size_t)slot * P.G + ib) * 3 + tid] = s;
+  }
+  if (tid == 0) {
+    *D.done = 0;
+    (void)kb;
+  }
+}
+
+// slice-averaged vertical centroid of every bucket (writeoutVCMmotionsPM2,
+// src/fbi_weakstrong.c:688-737)
+__global__ void k_ws_end(WSP P, WSD D, const int *bunch_of_bucket, int slot)
+{
+  const int kb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kb >= P.h) return;
+  double z = 0.0;
+  const int b = bunch_of_bucket[kb];
+  if (b >= 0) {
+    const double *zz = D.sl + (size_t)b * 6 * P.S + 4 * P.S;
+    for (int ic = 0; ic < P.S; ic++) z = z + zz[ic];
+    z = z / ((double)P.S);
+  }
+  D.zcm[(size_t)slot * P.h + kb] = z;
+}
+
+// ================================================================ host side
+
+struct mbt_ws {
+  mbt_config cfg;
+  int device;
+  hipStream_t stream;
+  WSP P;
+  WSD D;
+  int *d_bucket, *d_fill, *d_npop, *d_nofe, *d_bob;
+  double *d_Ibunch, *d_nelec, *d_convion;
+  int *bucket_h, *fill_h;
+  int npop_h[WS_S];
+  double *dist; // generated macro-electrons [6][Np]
+  double *nelec_h;
+  long long rev;
+  int maxblk;
+};
+
+extern "C" void mbt_ws_destroy(mbt_ws *w)
+{
+  if (!w) return;
+  (void)hipSetDevice(w->device);
+  void *p[] = {w->D.sl,  w->D.hist, w->D.Ex,   w->D.Ey,  w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
+               w->D.done, w->D.cm,  w->D.mon,  w->D.ion, w->D.zcm,    w->d_bucket, w->d_fill,    w->d_npop,
+               w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion};
+  for (size_t i = 0; i < sizeof p / sizeof p[0]; i++)
+    if (p[i]) (void)hipFree(p[i]);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  free(w->bucket_h);
+  free(w->fill_h);
+  free(w->dist);
+  free(w->nelec_h);
+  free(w);
+}
+
+template <typename T> static int walloc(T **p, size_t n)
+{
+  if (n == 0) n = 1;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+    mbt_set_error("hipMalloc of %zu bytes failed", n * sizeof(T));
+    return MBT_ENOMEM;
+  }
+  return hipMemset(*p, 0, n * sizeof(T)) == hipSuccess ? MBT_OK : MBT_EDEVICE;
+}
+
+#define WA(p, n)                                                                                 \
+  do {                                                                                           \
+    int rc_ = walloc(&(p), (size_t)(n));                                                         \
+    if (rc_) {                                                                                   \
+      mbt_ws_destroy(w);                                                                         \
+      return rc_;                                                                                \
+    }                                                                                            \
+  } while (0)
+#define WUP(dst, src, n) CK(hipMemcpyAsync((dst), (src), (size_t)(n) * sizeof(*(src)), hipMemcpyHostToDevice, w->stream))
+
+extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws **out)
+{
+  *out = NULL;
+  if (c->bunch_model != 1) {
+    mbt_set_error("mbt_ws_create needs BunchModel = STRONG");
+    return MBT_EINVAL;
+  }
+  const int S = 2 * c->Nslc + 1, H = (int)c->Nharm, G = c->Nbunch;
+  if (c->Nslc < 0 || S > WS_S || H > 500 || G <= 0 || c->NmultiT > WS_HIST || c->Nmlt + 2 > WS_HIST ||
+      c->NrevMon <= 0) {
+    mbt_set_error("weak-strong limits: 2 Nslc + 1 <= 50, h <= 500, Nmlt + 2 <= 30, NrevMon > 0");
+    return MBT_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
+    mbt_set_error("no HIP device %d", device);
+    return MBT_EDEVICE;
+  }
+  mbt_ws *w = (mbt_ws *)calloc(1, sizeof *w);
+  w->cfg = *c;
+  w->cfg.nfFill = NULL;
+  w->cfg.Ibunch = NULL;
+  w->cfg.bucket = NULL;
+  w->cfg.bunch_Ib = NULL;
+  w->cfg.Ib_frac = NULL;
+  w->cfg.bunch_out = NULL;
+  w->device = device;
+  CK(hipSetDevice(device));
+  CK(hipStreamCreate(&w->stream));
+  // ---- slices, populations, grid counts (host, the reference's generator)
+  double *slices = (double *)malloc(sizeof(double) * 6 * (size_t)S * H);
+  int nofe[WS_PTS];
+  w->dist = (double *)malloc(sizeof(double) * 6 * (size_t)(c->Np > 0 ? c->Np : 1));
+  int rc = mbt_ws_generate(c, slices, w->npop_h, nofe, w->dist);
+  if (rc) {
+    free(slices);
+    mbt_ws_destroy(w);
+    return rc;
+  }
+  w->bucket_h = (int *)malloc(sizeof(int) * G);
+  w->fill_h = (int *)malloc(sizeof(int) * H);
+  memcpy(w->bucket_h, c->bucket, sizeof(int) * G);
+  memcpy(w->fill_h, c->nfFill, sizeof(int) * H);
+  double *Ib = (double *)calloc(H, sizeof(double));
+  if (flags & MBT_WS_BUNCH_CURRENTS) memcpy(Ib, c->Ibunch, sizeof(double) * H);
+  // ---- constants (glibc on the host, the reference's expressions)
+  WSP &P = w->P;
+  P.S = S;
+  P.G = G;
+  P.h = H;
+  P.Nmlt = c->Nmlt;
+  P.NmultiT = c->NmultiT;
+  P.fbii = c->EnableFBII != 0;
+  P.rw = c->EnableRW_long != 0;
+  P.track_h = c->TrackPlane[1] != 0;
+  P.track_v = c->TrackPlane[2] != 0;
+  for (int p = 0; p < 3; p++) P.track[p] = c->TrackPlane[p] != 0;
+  P.Nions = S * 25;
+  P.eps = c->Vrf0 / (c->E0 * MBT_FKILO);
+  P.wrf = c->wrf;
+  P.phai0 = c->phai0;
+  P.Urad = c->Urad;
+  P.T0 = c->T0;
+  P.ac = c->ac;
+  for (int p = 0; p < 3; p++) {
+    P.gamma1[p] = c->gamma1[p];
+    P.alpha1[p] = c->alpha1[p];
+    P.beta1[p] = c->beta1[p];
+  }
+  {
+    double PsiV0 = 2.0 * M_PI * c->QV0 / 1.0, PsiH0 = 2.0 * M_PI * c->QH0 / 1.0;
+    double cv = cos(PsiV0), sv = sin(PsiV0), ch = cos(PsiH0), sh = sin(PsiH0);
+    P.v11 = cv + c->alpha1[2] * sv;
+    P.v12 = c->beta1[2] * sv;
+    P.v21 = -c->gamma1[2] * sv;
+    P.v22 = cv - c->alpha1[2] * sv;
+    P.h11 = ch + c->alpha1[1] * sh;
+    P.h12 = c->beta1[1] * sh;
+    P.h21 = -c->gamma1[1] * sh;
+    P.h22 = ch - c->alpha1[1] * sh;
+    for (int pl = 0; pl < 2; pl++) {
+      const int plane = pl + 1;
+      double Psi0 = 2.0 * M_PI * (plane == 1 ? c->QH0 : c->QV0) / 1.0;
+      double Psi = 0.0 + Psi0;
+      double cp = cos(Psi), sp = sin(Psi);
+      P.rwa11[pl] = cp + c->alpha1[plane] * sp;
+      P.rwa12[pl] = +c->beta1[plane] * sp;
+      const double kappa = 1.0 / 2.8e-08;
+      double beff3 = pow(plane == 1 ? c->beffH[0] : c->beffV[0], 3);
+      P.prfrw[pl] = 2.0 * c->R / (beff3) / sqrt(M_PI * kappa * K_EPS0) / 1.0;
+      beff3 = pow(2.0e-3, 3);
+      P.prfrw_trig[pl] = 2.0 * c->R / (beff3) / sqrt(M_PI * kappa * K_EPS0) / 1.0;
+    }
+  }
+  P.triggerRW = c->triggerRW;
+  P.Pref0 = c->T0 / c->E0 / c->fNp / MBT_FTERA;
+  P.fNp = c->fNp;
+  P.Dub0 = c->Lc / ((double)c->Nharm);
+  P.Lc = c->Lc;
+  P.factBE = sqrt(2 * (c->sgm_xx * c->sgm_xx - c->sgm_zz * c->sgm_zz)) / MBT_FKILO;
+  P.sxx = c->sgm_xx;
+  P.szz = c->sgm_zz;
+  P.deltT = 0.3 * c->sgmatau;
+  mbt_ws_grid(c, &P.aLx, &P.aLz);
+  P.aAx = 2.0 * P.aLx;
+  P.aAz = 2.0 * P.aLz;
+  P.den_dQ = 2 * K_EPS0 * sqrt(M_PI) * P.factBE * K_C * P.deltT;
+  {
+    double Pave = 2.7e-9, Pu20 = 5 * Pave, Pm = 0.15 * Pu20 / 1.3332, dm = 3.22e22 * Pm, sigmam = 1.78;
+    double Lu20 = 354.0, Ldmsgm = Lu20 * dm * sigmam * 1e-22;
+    P.factF = K_E / (28.0 * 1.836e3 * 9.1e-31);
+    P.factjp0 = 1e+2 * 4.0 * M_PI * K_EPS0 * 2.819e-15 / c->Gamma;
+    P.CdeltT = K_C * P.deltT;
+    w->nelec_h = (double *)malloc(sizeof(double) * G);
+    double *conv = (double *)malloc(sizeof(double) * G);
+    for (int b = 0; b < G; b++) {
+      w->nelec_h[b] = (Ib[c->bucket[b]] * c->T0 / (c->Np * K_E * MBT_FKILO));
+      conv[b] = w->nelec_h[b] * Ldmsgm;
+    }
+    // ---- device memory
+    const size_t npts = (size_t)S * 25;
+    WA(w->D.sl, (size_t)G * 6 * S);
+    WA(w->d_npop, S);
+    WA(w->d_nofe, npts);
+    WA(w->d_bucket, G);
+    WA(w->d_fill, H);
+    WA(w->d_bob, H);
+    WA(w->d_Ibunch, H);
+    WA(w->d_nelec, G);
+    WA(w->d_convion, G);
+    WA(w->D.hist, (size_t)4 * H * WS_HIST);
+    WA(w->D.Ex, (size_t)G * npts);
+    WA(w->D.Ey, (size_t)G * npts);
+    WA(w->D.li, (size_t)5 * G * npts);
+    w->D.cap = (long long)G * npts;
+    WA(w->D.cl, (size_t)5 * w->D.cap);
+    w->maxblk = (int)((w->D.cap + WS_WG - 1) / WS_WG);
+    WA(w->D.part, (size_t)w->maxblk * S * 2);
+    WA(w->D.psum, (size_t)w->maxblk * 3);
+    WA(w->D.done, 1);
+    WA(w->D.cm, (size_t)WS_T * G * 4);
+    WA(w->D.mon, (size_t)WS_T * G * 18);
+    WA(w->D.ion, (size_t)WS_T * G * 3);
+    WA(w->D.zcm, (size_t)WS_T * H);
+    double *sl = (double *)malloc(sizeof(double) * 6 * (size_t)S * G);
+    int *bob = (int *)malloc(sizeof(int) * H);
+    for (int kb = 0; kb < H; kb++) bob[kb] = -1;
+    for (int b = 0; b < G; b++) {
+      memcpy(sl + (size_t)b * 6 * S, slices + (size_t)c->bucket[b] * 6 * S, sizeof(double) * 6 * S);
+      bob[c->bucket[b]] = b;
+    }
+    WUP(w->D.sl, sl, (size_t)G * 6 * S);
+    WUP(w->d_npop, w->npop_h, S);
+    WUP(w->d_nofe, nofe, npts);
+    WUP(w->d_bucket, w->bucket_h, G);
+    WUP(w->d_fill, w->fill_h, H);
+    WUP(w->d_bob, bob, H);
+    WUP(w->d_Ibunch, Ib, H);
+    WUP(w->d_nelec, w->nelec_h, G);
+    WUP(w->d_convion, conv, G);
+    CK(hipStreamSynchronize(w->stream));
+    free(sl);
+    free(bob);
+    free(conv);
+  }
+  free(slices);
+  free(Ib);
+  w->D.npop = w->d_npop;
+  w->D.nofe = w->d_nofe;
+  w->D.bucket = w->d_bucket;
+  w->D.fill = w->d_fill;
+  w->D.Ibunch = w->d_Ibunch;
+  w->D.nelec = w->d_nelec;
+  w->D.convion = w->d_convion;
+  *out = w;
+  return MBT_OK;
+}
+
+static int ws_download(mbt_ws *w, long n, double *cm, double *mon, double *ion, double *zcm)
+{
+  CK(hipStreamSynchronize(w->stream));
+  const int G = w->P.G, H = w->P.h;
+  if (cm) CK(hipMemcpy(cm, w->D.cm, sizeof(double) * 4 * G * n, hipMemcpyDeviceToHost));
+  if (mon) CK(hipMemcpy(mon, w->D.mon, sizeof(double) * 18 * G * n, hipMemcpyDeviceToHost));
+  if (ion) CK(hipMemcpy(ion, w->D.ion, sizeof(double) * 3 * G * n, hipMemcpyDeviceToHost));
+  if (zcm) CK(hipMemcpy(zcm, w->D.zcm, sizeof(double) * H * n, hipMemcpyDeviceToHost));
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, double *ion, double *zcm)
+{
+  CK(hipSetDevice(w->device));
+  (void)hipGetLastError();
+  const int G = w->P.G, H = w->P.h;
+  const size_t lds2 = sizeof(double) * 2 * H;
+  long done = 0;
+  while (done < nturns) {
+    long n = nturns - done;
+    if (n > WS_T) n = WS_T;
+    CK(hipMemsetAsync(w->D.mon, 0, sizeof(double) * 18 * G * n, w->stream));
+    CK(hipMemsetAsync(w->D.ion, 0, sizeof(double) * 3 * G * n, w->stream));
+    for (long t = 0; t < n; t++) {
+      const long long rev = w->rev;
+      const int monitor = rev % w->cfg.NrevMon == 0;
+      hipLaunchKernelGGL(k_ws_phase1, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, rev, (int)t, monitor);
+      CK(hipGetLastError());
+      hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
+      CK(hipGetLastError());
+      if (w->P.fbii) {
+        hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
+        CK(hipGetLastError());
+        for (int ib = 1; ib < G; ib++) {
+          const long long nF = (long long)ib * w->P.Nions;
+          const double Dtime =
+              fabs((double)(w->bucket_h[ib] - w->bucket_h[ib - 1])) * (w->cfg.T0 / w->cfg.h);
+          const int nblk = (int)((nF + WS_WG - 1) / WS_WG);
+          hipLaunchKernelGGL(k_ws_train, dim3(nblk), dim3(WS_WG), 0, w->stream, w->P, w->D, ib, nF, Dtime, (int)t);
+          CK(hipGetLastError());
+        }
+      }
+      hipLaunchKernelGGL(k_ws_end, dim3((H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t);
+      CK(hipGetLastError());
+      w->rev++;
+    }
+    int rc = ws_download(w, n, cm ? cm + (size_t)done * G * 4 : NULL, mon ? mon + (size_t)done * G * 18 : NULL,
+                         ion ? ion + (size_t)done * G * 3 : NULL, zcm ? zcm + (size_t)done * H : NULL);
+    if (rc) return rc;
+    done += n;
+  }
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_get_slices(const mbt_ws *w, int ib, double *out)
+{
+  if (ib < 0 || ib >= w->P.G) return MBT_EINVAL;
+  CK(hipSetDevice(w->device));
+  CK(hipStreamSynchronize(w->stream));
+  CK(hipMemcpy(out, w->D.sl + (size_t)ib * 6 * w->P.S, sizeof(double) * 6 * w->P.S, hipMemcpyDeviceToHost));
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_population(const mbt_ws *w, int *npop, int *nslices)
+{
+  for (int i = 0; i < w->P.S; i++) npop[i] = w->npop_h[i];
+  if (nslices) *nslices = w->P.S;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_distribution(const mbt_ws *w, double *dist)
+{
+  memcpy(dist, w->dist, sizeof(double) * 6 * (size_t)w->cfg.Np);
+  return MBT_OK;
+}
+
+extern "C" long mbt_ws_turn(const mbt_ws *w) { return (long)w->rev; }

@@ -27,7 +27,15 @@
  *   zcm.txt   turn  z CM of every bucket (writeoutVCMmotionsPM2's value before scaling)
  *   slices.txt  ib cell xtau xeps xx xp zz zp npop   (after the last turn)
  *
- * Usage: refwsharness input.conf outdir [nturns] [ibunch_fill]
+ * With write_files = 1 the manager's own writers also run, at the points where
+ * manager_weakstrong calls them, and write its files into outdir:
+ * bunch_strong_writeout_ampinv (monitor turns), writeoutCMImotions (after every
+ * bunch of the FBII relay), writeoutVCMmotionsPM2 (every turn),
+ * bunch_writeout_mean_ampinv_pos (end), and its progress lines go to stdout.
+ * (The MPI program itself cannot run this mode below h = 500: its workers
+ * receive 500 doubles into ring.Ibunch, allocated with h entries.)
+ *
+ * Usage: refwsharness input.conf outdir [nturns] [ibunch_fill] [write_files]
  * ibunch_fill = 1 sets ring.Ibunch from the filling as the weak-weak manager
  * does (src/manager_weakweak.c:58-80); 0 keeps the reference's zeros.
  */
@@ -74,6 +82,7 @@ int main(int argc, char **argv)
   }
   long nturns_override = argc > 3 ? atol(argv[3]) : -1;
   int ib_fill = argc > 4 ? atoi(argv[4]) : 0;
+  int wfiles = argc > 5 ? atoi(argv[5]) : 0;
   char conf[FILENAME_MAX];
   if (argv[1][0] == '/') snprintf(conf, sizeof conf, "%s", argv[1]);
   else {
@@ -123,6 +132,10 @@ int main(int argc, char **argv)
   bunch_CM_history_strong_init(wh, (int)(track.NrevTot / track.NrevMon * ebeam.Nbunch));
   bunch_strong_t bunch0;
   memset(&bunch0, 0, sizeof bunch0);
+  if (wfiles) {
+    printf("Starting job \"%s\"\n\n", track.jobtitle);
+    printf("Generating initial distribution...\n");
+  }
   if (strong_generate_eCMs(mp, &ebeam, &bunch0, 1) < 0) return 1;
   bunch_strong_t *bunches = calloc(500, sizeof(bunch_strong_t));
   if (generate_initial_general_distribution2(bunches, &bunch0, mp, ring, ebeam) < 0) return 1;
@@ -150,7 +163,13 @@ int main(int argc, char **argv)
   FILE *fcm = fopen("cm.txt", "w"), *fmon = fopen("mon.txt", "w"), *fion = fopen("ion.txt", "w");
   FILE *fz = fopen("zcm.txt", "w");
   (void)path;
+  if (wfiles) printf("Starting multi bunch tracking...\n");
   for (long rev = 0; rev < track.NrevTot; rev++) {
+    if (wfiles && rev % 100 == 0) {
+      if (rev + 99 >= track.NrevTot) printf("Transformation for turn %3ld to %3ld", rev, track.NrevTot - 1);
+      else printf("Transformation for turn %3ld to %3ld", rev, rev + 99);
+      fflush(stdout);
+    }
     int ib = 0;
     for (int kb = 0; kb < h; kb++) {
       if (!ebeam.nfFill[kb]) continue;
@@ -179,11 +198,25 @@ int main(int argc, char **argv)
         put3(fmon, &b.stats.ampinv);
         put3(fmon, &b.stats.ampinv_cm);
         fprintf(fmon, "\n");
+        /* the manager's store (src/manager_weakstrong.c:130-140); its arrays hold
+           NrevTot/NrevMon*Nbunch + 2 entries, which bucket-indexed writes can
+           pass when buckets outnumber bunches: those are skipped here */
+        const int irevmon = rev / track.NrevMon;
+        const long nout = (long)(track.NrevTot / track.NrevMon * ebeam.Nbunch) + 2;
+        if (wfiles && irevmon * ebeam.Nbunch + kb < nout) {
+        mh->cm[irevmon * ebeam.Nbunch + kb] = b.stats.pos;
+        mh->rms[irevmon * ebeam.Nbunch + kb] = b.stats.pos_sigma;
+        mh->cm_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope;
+        mh->rms_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope_sigma;
+        mh->ampinv[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv;
+        mh->ampinv_cm[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv_cm;
+        }
       }
       wb[kb] = b;
       bunches[kb] = b;
       ib++;
     }
+    if (wfiles && rev % track.NrevMon == 0) bunch_strong_writeout_ampinv(rev, rev / track.NrevMon, mh, ebeam);
     sortCMdata((int)rev, mh);
     memcpy(wh->dataXX, mh->dataXX, 5000 * sizeof(double));
     memcpy(wh->dataXP, mh->dataXP, 5000 * sizeof(double));
@@ -220,7 +253,14 @@ int main(int argc, char **argv)
       }
       fprintf(fion, "%ld %d %.17g %.17g %.17g\n", rev, ib, sx, sy, sn);
       bunches[kb] = wb[kb];
+      if (wfiles && track.EnableFBII) writeoutCMImotions(track.work_path, (int)rev, ib, ionbeam, &ebeam, mp);
       ib++;
+    }
+    if (wfiles) {
+      writeoutVCMmotionsPM2(track.work_path, (int)rev, bunches, &ebeam);
+      if (rev % 10 == 9) printf(".");
+      fflush(stdout);
+      if (rev % 100 == 99 || rev == track.NrevTot - 1) printf(" done\n");
     }
     fprintf(fz, "%ld", rev);
     for (int kb = 0; kb < h; kb++) {
@@ -232,6 +272,10 @@ int main(int argc, char **argv)
       fprintf(fz, " %.17g", z);
     }
     fprintf(fz, "\n");
+  }
+  if (wfiles) {
+    if (track.TrackPlane[HOR]) bunch_writeout_mean_ampinv_pos(track.NrevTot, track.NrevMon, mh, ebeam, HOR);
+    if (track.TrackPlane[VER]) bunch_writeout_mean_ampinv_pos(track.NrevTot, track.NrevMon, mh, ebeam, VER);
   }
   fclose(fcm);
   fclose(fmon);

@@ -71,7 +71,7 @@ def make(name):
 
 # per-variant section overrides of the end-to-end cases (none at present)
 E2E_OVERRIDES = {}
-E2E_CASES = ["wakes", "lr_fb", "rsis"]
+E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform"]
 
 MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"
@@ -83,6 +83,8 @@ def make_e2e(name="wakes"):
     tests/golden/e2e_<name>/ for the drop-in driver's file-level comparison."""
     import shutil
     out = os.path.join(HERE, "e2e_" + name)
+    if name.startswith("ws_"):
+        return make_e2e_ws(name)
     with tempfile.TemporaryDirectory() as d:
         conf = write_variant(name, d, **E2E_OVERRIDES.get(name, {}))
         work = os.path.join(d, "work")
@@ -114,6 +116,33 @@ def make_e2e(name="wakes"):
 
 
 WS_HARNESS = os.path.join(ROOT, "oracle", "_ref", "refwsharness")
+
+
+def make_e2e_ws(name):
+    """The weak-strong manager's files for a STRONG configuration, written by
+    the reference's own writer functions through refwsharness (the MPI program
+    cannot run this mode below h = 500, see oracle/ref_ws_harness.c)."""
+    import re
+    import shutil
+    from variants import write_ws_variant
+    out = os.path.join(HERE, "e2e_" + name)
+    keep_out = {"cm.txt", "mon.txt", "ion.txt", "zcm.txt", "slices.txt"}
+    with tempfile.TemporaryDirectory() as d:
+        conf, mode = write_ws_variant(name, d)
+        work = os.path.join(d, "work")
+        os.makedirs(work)
+        r = subprocess.run([WS_HARNESS, conf, work, "0", str(mode), "1"], capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        if os.path.exists(out):
+            shutil.rmtree(out)
+        os.makedirs(out)
+        for f in sorted(os.listdir(work)):
+            if f not in keep_out:
+                shutil.copy(os.path.join(work, f), os.path.join(out, f))
+        with open(os.path.join(out, "stdout.txt"), "w") as f:
+            f.write(re.sub(r"at time: [0-9:]+", "at time: T", r.stdout))
+    return out
 WOFZ_DUMP = os.path.join(ROOT, "oracle", "_ref", "refwofz")
 
 

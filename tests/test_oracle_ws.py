@@ -99,3 +99,20 @@ def test_ws_live_harness_matches_fixture(tmp_path):
     assert r.returncode == 0
     g = np.load(os.path.join(GOLD, "ws_fbii.npz"))
     assert np.array_equal(np.loadtxt(os.path.join(tmp_path, "cm.txt"), ndmin=2), g["cm"])
+
+
+@pytest.mark.parametrize("name", ["ws_fbii", "ws_h416"])
+def test_product_setup_equals_oracle(tmp_path, name):
+    """The product's host set-up (mbt_ws_generate, an independent restatement)
+    gives the oracle's slice centroids, populations, grid counts and
+    macro-electrons bit for bit (no GPU needed)."""
+    path, mode = write_ws_variant(name, str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    sl, npop, nofe, dist = m.ws_generate(cfg)
+    w = o.OracleWS(cfg, mode)
+    onpop, onofe = w.population()
+    assert np.array_equal(npop, onpop) and np.array_equal(nofe, onofe)
+    assert np.array_equal(dist, w.dist)
+    for ib, kb in enumerate(cfg.buckets):
+        assert np.array_equal(sl[kb], w.slices(ib)), (ib, kb)
+    w.close()

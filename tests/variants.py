@@ -91,10 +91,10 @@ def write_variant(name, d, **section_overrides):
 WS_FILL = {20: [int(k * 20 / 12) for k in range(12)], 416: [int(k * 416 / 12) for k in range(12)]}
 
 
-def _ws(h, fbii, rw, nmlt=3, turns=30, np_=2000):
+def _ws(h, fbii, rw, nmlt=3, turns=30, np_=2000, filling="@FILL@"):
     return dict(sections={
         "tracking": {"BunchModel": "STRONG", "EnableFBII": fbii, "NrevTot": turns, "NrevMon": 10,
-                     "EnableRW_long": rw, "Nmlt": nmlt, "EnableQuantum": 0, "filling": "@FILL@"},
+                     "EnableRW_long": rw, "Nmlt": nmlt, "EnableQuantum": 0, "filling": filling},
         "ring": dict(Lc=cg.small_ring(h), Iring=100.0, **RW),
         "distribution": {"Np": np_, "Nslc": 20}})
 
@@ -106,6 +106,9 @@ WS_VARIANTS = {
     "ws_fbii_refcur": (_ws(20, 1, 1), 0),
     "ws_optics": (_ws(20, 0, 0), 1),
     "ws_h416": (_ws(416, 1, 1, nmlt=12), 1),
+    # every bucket filled (the reference program's manager addresses bucket kb
+    # as rank kb + 1, so its end-to-end run needs buckets 0..Nbunch-1 filled)
+    "ws_uniform": (_ws(20, 1, 1, filling="UNIFORM"), 0),
 }
 
 
