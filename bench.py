@@ -20,7 +20,10 @@ a bounded number of turns.
 `batched` is the ensemble workload: 256 independent rings (seeded draws of
 bunch count, macroparticle count, current and chromaticity; RF beam-loading
 feedback on) tracked together by the same kernels, FP64, with its own
-throughput and k_turn roofline.
+throughput and k_turn roofline; the work decomposition is chosen by the
+autotuner (mbt_ring_autotune) first.  `fbii` is the weak-strong fast beam-ion
+workload (356-bunch train in h = 416), in ion-slice interactions/s, with the
+reference's own functions timed on a 48-bunch train beside it.
 """
 import argparse
 import json
@@ -105,6 +108,7 @@ def batched(steps, warmup, nrings, seed):
                 ens.set_bunch(g, b0)
         ens.prepare()
         R, G, N = ens.size()
+        chunk, tune_ms = ens.autotune()
         if warmup > 0:
             ens.track(warmup)
         g0 = time.perf_counter()
@@ -126,13 +130,72 @@ def batched(steps, warmup, nrings, seed):
                                    "32-416 and macroparticles 1e4-1e5 log-uniform, 0.1-5 mA/bunch, chromaticity "
                                    "-1..3, 100 slices, broadband resonator + resistive wall 50 turns + main cavity "
                                    "with RF beam-loading feedback, QE off (seed %d)" % (R, seed),
-                       "rings": R, "bunches": G, "macroparticles": N},
+                       "rings": R, "bunches": G, "macroparticles": N,
+                       "chunk_items": chunk, "autotune_ms": tune_ms},
             "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                    "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None},
             "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
                                "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)}}
+
+
+def fbii_conf(d, train=356, np_=10000, turns=2000):
+    """Fast beam-ion weak-strong workload: h = 416, a train of `train` bunches
+    in buckets 0..train-1 followed by the gap, 2 Nslc + 1 = 41 slice centroids
+    per bunch, 41 x 25 = 1,025 macro-ions created per bunch passage on the
+    5 x 5 local grid of every slice, Bassetti-Erskine fields through the
+    Faddeeva function, ion cloud cleared at the head of the train, FP64."""
+    from mtrack_amd import confgen
+    fill = os.path.join(d, "fill_fbii.txt")
+    confgen.filling_file(fill, list(range(train)), out=())
+    text = confgen.ring_conf(sections={
+        "tracking": {"BunchModel": "STRONG", "EnableFBII": 1, "NrevTot": turns, "NrevMon": 100,
+                     "EnableRW_long": 0, "EnableQuantum": 0, "filling": fill},
+        "distribution": {"Np": np_, "Nslc": 20}})
+    path = os.path.join(d, "fbii.conf")
+    confgen.write(path, text)
+    return path
+
+
+def ion_slice_updates(nb, per_bunch=1025, slices=41):
+    """Ion-slice interactions of one turn's train: bunch k meets the k*1025 ions
+    of the bunches ahead of it at each of its 41 slices."""
+    return sum(k * per_bunch for k in range(nb)) * slices
+
+
+def fbii(steps, warmup, cpu=True):
+    import mtrack_amd as m
+    with tempfile.TemporaryDirectory() as d:
+        cfg = m.Config(fbii_conf(d), d + "/")
+        w = m.WeakStrong(cfg, bunch_currents=True)
+        nb = w.Nb
+        if warmup > 0:
+            w.track(warmup)
+        t0 = time.perf_counter()
+        w.track(steps)
+        t1 = time.perf_counter()
+        w.close()
+    sec = (t1 - t0) / steps
+    out = {"value": ion_slice_updates(nb) / sec, "unit": "ion-slice interactions/s", "dtype": "f64",
+           "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * sec,
+           "config": {"workload": "fast beam-ion weak-strong: h 416, %d-bunch train + %d-bucket gap, 41 slice "
+                                  "centroids and 1,025 macro-ions per bunch passage, Bassetti-Erskine fields, "
+                                  "bunch currents of the filling" % (nb, 416 - nb),
+                      "bunches": nb, "ion_slice_interactions_per_turn": ion_slice_updates(nb)}}
+    if cpu:
+        harness = os.path.join(ROOT, "oracle", "_ref", "refwsharness")
+        if os.path.exists(harness):
+            with tempfile.TemporaryDirectory() as d:
+                conf = fbii_conf(d, train=48, turns=1)
+                t0 = time.perf_counter()
+                r = subprocess.run([harness, conf, d, "1", "1"], capture_output=True, text=True, timeout=300)
+                t1 = time.perf_counter()
+                if r.returncode == 0:
+                    out["cpu_baseline"] = {"value": ion_slice_updates(48) / (t1 - t0), "unit": "ion-slice interactions/s",
+                                           "cores": 1, "kind": "reference",
+                                           "sample": "48-bunch train, 1 turn, set-up included"}
+    return out
 
 
 def copy_calibration():
@@ -161,6 +224,8 @@ def main():
     ap.add_argument("--batched-rings", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-seed", type=int, default=20240607)
+    ap.add_argument("--no-fbii", action="store_true", help="skip the fast beam-ion line")
+    ap.add_argument("--fbii-steps", type=int, default=10)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
@@ -223,6 +288,11 @@ def main():
             out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed)
         except Exception as e:  # reported beside the FP64 line, never replaces it
             out["batched"] = {"value": None, "error": str(e)[:300]}
+    if not a.no_fbii:
+        try:
+            out["fbii"] = fbii(a.fbii_steps, 2, cpu=not a.no_cpu_baseline)
+        except Exception as e:
+            out["fbii"] = {"value": None, "error": str(e)[:300]}
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(a.np_, a.cpu_turns)

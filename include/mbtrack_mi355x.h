@@ -227,6 +227,15 @@ double mbt_ring_bytes_per_turn(const mbt_ring *r);
 /* Per-kernel timing with HIP events around every launch group (0 = off). */
 int mbt_ring_set_timing(mbt_ring *r, int on);
 long mbt_ring_turn(const mbt_ring *r);
+/* Work decomposition of the per-macroparticle pass: chunks of 256 * items
+ * macroparticles of one bunch per work-group, items in [4, 64] (default 8).
+ * No counterpart in the reference (a tuning knob of this build); changes results
+ * only at the rounding level (grouping of the moment sums). */
+int mbt_ring_set_chunk(mbt_ring *r, int items);
+int mbt_ring_get_chunk(const mbt_ring *r);
+/* Times the statistics pass for items = 4, 8, 16, 32 (ms[4], may be NULL),
+ * keeps the fastest and returns it in *best_items.  Call after mbt_ring_prepare. */
+int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
 
 /* ---- ensembles of independent rings on one GPU ------------------------------ */
 /* No counterpart in the reference, which runs one ring per MPI job: nrings

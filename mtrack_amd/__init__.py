@@ -63,6 +63,9 @@ def lib():
         L.mbt_ensemble_set_bunch.argtypes = [vp, ip, dp]
         L.mbt_ensemble_get_bunch.argtypes = [vp, ip, dp]
         L.mbt_ensemble_size.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_longlong)]
+        L.mbt_ring_set_chunk.argtypes = [vp, ip]
+        L.mbt_ring_get_chunk.argtypes = [vp]
+        L.mbt_ring_autotune.argtypes = [vp, ctypes.POINTER(ip), dp]
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
         L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
@@ -201,6 +204,19 @@ class Ring:
 
     def bytes_per_turn(self):
         return lib().mbt_ring_bytes_per_turn(self.ptr)
+
+    def set_chunk(self, items):
+        _check(lib().mbt_ring_set_chunk(self.ptr, items))
+
+    def chunk(self):
+        return lib().mbt_ring_get_chunk(self.ptr)
+
+    def autotune(self):
+        """Chooses the per-work-group chunk by timing; returns (items, {items: ms})."""
+        best = ctypes.c_int()
+        ms = np.zeros(4)
+        _check(lib().mbt_ring_autotune(self.ptr, ctypes.byref(best), dptr(ms)))
+        return best.value, dict(zip((4, 8, 16, 32), ms.tolist()))
 
 
 class Ensemble(Ring):

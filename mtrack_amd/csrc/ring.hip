@@ -44,6 +44,10 @@
 #define TURN_WAVES_PER_EU 4
 #endif
 #define STATS_TURNS 256
+#define MIN_CHUNK_ITEMS 4
+#define AUTOTUNE_CANDIDATES 4
+#define MAX_CHUNK_ITEMS 64
+#define LR_PREFETCH 16
 #ifndef COORD_NONTEMPORAL
 #define COORD_NONTEMPORAL 1
 #endif
@@ -1031,8 +1035,8 @@ __global__ void k_lr(int nres, const LRConst *lc, const DevParams *__restrict__ 
   const double *fnp = fnp_ring + (size_t)L.plane * plane_stride + (size_t)L.bucket_off * Nc;
   const double *Ib = Ibunch + L.bucket_off;
   const int *fill = nfFill + L.bucket_off;
-  double *out = lrc + L.lrc_off;
-  double *fbh = fb_hist + P.fbh_off;
+  double *__restrict__ out = lrc + L.lrc_off;
+  double *__restrict__ fbh = fb_hist + P.fbh_off;
   const int fb_len = P.fb_len;
   const int nturn = init ? L.init_turns : 1;
   if (nturn <= 0) return;
@@ -1060,14 +1064,23 @@ __global__ void k_lr(int nres, const LRConst *lc, const DevParams *__restrict__ 
       }
       if (fill[i] == 1) {
         double Ii = Ib[i];
-        const double *f = fnp + (size_t)i * Nc;
-        double *o = out + (size_t)i * Nc;
-        for (int j = 0; j < Nc; j++) {
-          Vn0 = (Vo0 * L.pr0 - Vo1 * L.pr1) - Ii * f[j] * L.fac;
-          Vn1 = (Vo0 * L.pr1 + Vo1 * L.pr0);
-          if (!init) o[j] = Vn0 + 0.5 * Ii * f[j] * L.fac;
-          Vo0 = Vn0;
-          Vo1 = Vn1;
+        const double *__restrict__ f = fnp + (size_t)i * Nc;
+        double *__restrict__ o = out + (size_t)i * Nc;
+        // the recurrence is latency-bound: fetch the bucket's slice values a
+        // block ahead of the dependent chain
+        for (int j0 = 0; j0 < Nc; j0 += LR_PREFETCH) {
+          double fv[LR_PREFETCH];
+#pragma unroll
+          for (int k2 = 0; k2 < LR_PREFETCH; k2++) fv[k2] = j0 + k2 < Nc ? f[j0 + k2] : 0.0;
+#pragma unroll
+          for (int k2 = 0; k2 < LR_PREFETCH; k2++) {
+            if (j0 + k2 >= Nc) break;
+            Vn0 = (Vo0 * L.pr0 - Vo1 * L.pr1) - Ii * fv[k2] * L.fac;
+            Vn1 = (Vo0 * L.pr1 + Vo1 * L.pr0);
+            if (!init) o[j0 + k2] = Vn0 + 0.5 * Ii * fv[k2] * L.fac;
+            Vo0 = Vn0;
+            Vo1 = Vn1;
+          }
         }
         Vn0 = (Vo0 * L.ge * L.gc - Vo1 * L.ge * L.gs);
         Vn1 = (Vo0 * L.ge * L.gs + Vo1 * L.ge * L.gc);
@@ -1151,7 +1164,8 @@ struct mbt_ring {
   hipStream_t stream;
   int G, H, Nc;    // bunches, buckets (all rings), slices
   long long Ntot;
-  int nwg;
+  int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
+  int chunk_items;   // macroparticles per lane of one work-group's chunk
   int any_qe, any_dip, any_gen, nrev_mon;
   int sf_any;
   Coords C;
@@ -1391,6 +1405,71 @@ static int check_cfg(const mbt_config *c)
   return MBT_OK;
 }
 
+static int wg_count(const mbt_ring *r, int items)
+{
+  const long long CH = (long long)WG * items;
+  long long n = 0;
+  for (int k = 0; k < r->R; k++) n += ((long long)r->rh[k].cfg.Np + CH - 1) / CH * r->rh[k].cfg.Nbunch;
+  return (int)n;
+}
+
+static void drop_graphs(mbt_ring *r)
+{
+  for (int i = 0; i < NGRAPH; i++)
+    if (r->graphs[i].exec) {
+      (void)hipGraphExecDestroy(r->graphs[i].exec);
+      r->graphs[i].exec = nullptr;
+    }
+}
+
+// Work-group decomposition: every bunch cut into contiguous chunks of
+// WG * items macroparticles, one work-group each.
+static int set_decomposition(mbt_ring *r, int items)
+{
+  if (items < MIN_CHUNK_ITEMS || items > MAX_CHUNK_ITEMS) {
+    mbt_set_error("chunk of %d items per lane outside [%d, %d]", items, MIN_CHUNK_ITEMS, MAX_CHUNK_ITEMS);
+    return MBT_EINVAL;
+  }
+  const int nwg = wg_count(r, items), G = r->G;
+  const long long CH = (long long)WG * items;
+  long long *wf = (long long *)malloc(nwg * sizeof(long long)), *we = (long long *)malloc(nwg * sizeof(long long));
+  int *wb = (int *)malloc(nwg * sizeof(int)), *b0 = (int *)malloc((G + 1) * sizeof(int));
+  int w = 0, g = 0;
+  long long pos = 0;
+  for (int k = 0; k < r->R; k++) {
+    const long long Np = r->rh[k].cfg.Np, per_b = (Np + CH - 1) / CH;
+    for (int bb = 0; bb < r->rh[k].cfg.Nbunch; bb++, g++) {
+      b0[g] = w;
+      for (long long kk = 0; kk < per_b; kk++, w++) {
+        wf[w] = pos + kk * CH;
+        we[w] = pos + ((kk + 1) * CH < Np ? (kk + 1) * CH : Np);
+        wb[w] = g;
+      }
+      pos += Np;
+    }
+  }
+  b0[G] = w;
+  int rc = MBT_OK;
+  if (hipMemcpyAsync(r->wg_first, wf, nwg * sizeof(long long), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+      hipMemcpyAsync(r->wg_end, we, nwg * sizeof(long long), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+      hipMemcpyAsync(r->wg_bunch, wb, nwg * sizeof(int), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+      hipMemcpyAsync(r->bunch_wg0, b0, (G + 1) * sizeof(int), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+      hipStreamSynchronize(r->stream) != hipSuccess) {
+    mbt_set_error("upload of the work-group decomposition failed");
+    rc = MBT_EDEVICE;
+  }
+  free(wf);
+  free(we);
+  free(wb);
+  free(b0);
+  if (rc == MBT_OK) {
+    r->nwg = nwg;
+    r->chunk_items = items;
+    drop_graphs(r); // cached graphs hold the old grid
+  }
+  return rc;
+}
+
 static int create_rings(const mbt_config *const *cfgs, int R, int device, int flags, mbt_ring **out)
 {
   *out = NULL;
@@ -1468,44 +1547,34 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   r->rw_total = rw_total;
   r->st_total = st_total;
   r->fbh_total = fbh_total;
-  // ---- workgroup decomposition: contiguous chunks of one bunch
-  const long long CH = (long long)WG * CHUNK_ITEMS;
-  int nwg = 0;
-  for (int k = 0; k < R; k++) nwg += (int)(((long long)r->rh[k].cfg.Np + CH - 1) / CH) * r->rh[k].cfg.Nbunch;
-  r->nwg = nwg;
-  long long *wf = (long long *)malloc(nwg * sizeof(long long)), *we = (long long *)malloc(nwg * sizeof(long long));
-  int *wb = (int *)malloc(nwg * sizeof(int)), *b0 = (int *)malloc((G + 1) * sizeof(int));
+  // ---- workgroup decomposition (contiguous chunks of one bunch): buffers sized
+  // for the smallest chunk, the decomposition itself set below
+  const int nwg = wg_count(r, MIN_CHUNK_ITEMS);
+  r->nwg_cap = nwg;
   int *nph = (int *)malloc(G * sizeof(int)), *bring = (int *)malloc(G * sizeof(int)), *bk = (int *)malloc(G * sizeof(int));
   long long *bfirst = (long long *)malloc(G * sizeof(long long));
   int *qring = (int *)malloc(H * sizeof(int)), *fill = (int *)malloc(H * sizeof(int));
   double *Ibq = (double *)malloc(H * sizeof(double)), *bIb = (double *)malloc(G * sizeof(double));
   {
-    int w = 0, g = 0, q = 0;
+    int g = 0, q = 0;
     long long pos = 0;
     for (int k = 0; k < R; k++) {
       const mbt_config *c = &r->rh[k].cfg;
-      const long long Np = c->Np, per_b = (Np + CH - 1) / CH;
+      const long long Np = c->Np;
       for (int i = 0; i < (int)c->Nharm; i++, q++) {
         qring[q] = k;
         fill[q] = c->nfFill[i];
         Ibq[q] = c->Ibunch[i];
       }
       for (int bb = 0; bb < c->Nbunch; bb++, g++) {
-        b0[g] = w;
         nph[g] = (int)Np;
         bring[g] = k;
         bk[g] = c->bucket[bb];
         bIb[g] = c->bunch_Ib[bb];
         bfirst[g] = pos;
-        for (long long kk = 0; kk < per_b; kk++, w++) {
-          wf[w] = pos + kk * CH;
-          we[w] = pos + ((kk + 1) * CH < Np ? (kk + 1) * CH : Np);
-          wb[w] = g;
-        }
         pos += Np;
       }
     }
-    b0[G] = w;
   }
   DA(r->C.tau, Ntot);
   DA(r->C.del, Ntot);
@@ -1619,10 +1688,6 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   DA(r->lrc, lrc_total > 0 ? lrc_total : 1);
   DA(r->phasor_end, (size_t)2 * (nres > 0 ? nres : 1));
   // ---- uploads
-  UP(r->wg_first, wf, nwg);
-  UP(r->wg_end, we, nwg);
-  UP(r->wg_bunch, wb, nwg);
-  UP(r->bunch_wg0, b0, G + 1);
   UP(r->bunch_first, bfirst, G);
   UP(r->np, nph, G);
   UP(r->bunch_ring, bring, G);
@@ -1670,10 +1735,6 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     CK(hipStreamSynchronize(r->stream));
     free(ts0);
   }
-  free(wf);
-  free(we);
-  free(wb);
-  free(b0);
   free(nph);
   free(bring);
   free(bk);
@@ -1738,6 +1799,11 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     mbt_set_error("device synchronisation after set-up failed");
     mbt_ring_destroy(r);
     return MBT_EDEVICE;
+  }
+  int rcd = set_decomposition(r, CHUNK_ITEMS);
+  if (rcd) {
+    mbt_ring_destroy(r);
+    return rcd;
   }
   *out = r;
   return MBT_OK;
@@ -2147,5 +2213,69 @@ extern "C" int mbt_ring_set_timing(mbt_ring *r, int on)
     for (int i = 0; i < 6 * STATS_TURNS; i++) CK(hipEventCreate(&r->evpool[i]));
   }
   r->timing = on;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ring_set_chunk(mbt_ring *r, int items)
+{
+  CK(hipSetDevice(r->device));
+  CK(hipStreamSynchronize(r->stream));
+  return set_decomposition(r, items);
+}
+
+extern "C" int mbt_ring_get_chunk(const mbt_ring *r) { return r->chunk_items; }
+
+// Chooses the chunk (macroparticles per lane of a work-group) by timing the
+// statistics pass of k_turn (it reads every coordinate, bins and reduces like
+// the turn pass, and changes no state but the histogram accumulators, cleared
+// afterwards) for each candidate; keeps the fastest.  The choice changes only
+// how the moment sums are grouped, i.e. results at the rounding level.
+extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
+{
+  static const int cand[AUTOTUNE_CANDIDATES] = {4, 8, 16, 32};
+  CK(hipSetDevice(r->device));
+  (void)hipGetLastError();
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  const turn_kernel_t k = r->any_dip ? k_turn<MODE_STATS, false, true, true, false>
+                                     : k_turn<MODE_STATS, false, false, false, false>;
+  const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
+  int best = r->chunk_items, rc = MBT_OK;
+  double best_ms = 1e300;
+  for (int c = 0; c < AUTOTUNE_CANDIDATES && rc == MBT_OK; c++) {
+    if ((rc = set_decomposition(r, cand[c]))) break;
+    float t = 0.0f;
+    for (int rep = 0; rep < 4; rep++) {
+      if (rep == 1) CK(hipEventRecord(e0, r->stream));
+      hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
+                         r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
+                         r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
+      CK(hipGetLastError());
+    }
+    CK(hipEventRecord(e1, r->stream));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&t, e0, e1));
+    const double m = t / 3.0;
+    if (ms) ms[c] = m;
+    if (m < best_ms) {
+      best_ms = m;
+      best = cand[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  // the statistics passes binned into the accumulators: clear them
+  const size_t GN = (size_t)r->G * (r->Nc > 0 ? r->Nc : 1);
+  CK(hipMemsetAsync(r->Hs.cnt, 0, GN * sizeof(unsigned), r->stream));
+  CK(hipMemsetAsync(r->Hs.sx, 0, GN * 2 * sizeof(unsigned long long), r->stream));
+  CK(hipMemsetAsync(r->Hs.sz, 0, GN * 2 * sizeof(unsigned long long), r->stream));
+  CK(hipMemsetAsync(r->Hs.tcnt, 0, (size_t)r->G * 2 * sizeof(unsigned), r->stream));
+  CK(hipMemsetAsync(r->Hs.tsx, 0, (size_t)r->G * 4 * sizeof(unsigned long long), r->stream));
+  CK(hipMemsetAsync(r->Hs.tsz, 0, (size_t)r->G * 4 * sizeof(unsigned long long), r->stream));
+  CK(hipMemsetAsync(r->Hs.bad, 0, (size_t)r->G * sizeof(int), r->stream));
+  if ((rc = set_decomposition(r, best))) return rc;
+  if (best_items) *best_items = best;
   return MBT_OK;
 }

@@ -98,3 +98,27 @@ def test_deterministic_bitwise(tmp_path):
         g.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("items", [4, 32])
+def test_chunk_sizes_match_oracle(tmp_path, items):
+    """Any work decomposition of the per-macroparticle pass gives the oracle's
+    turns to the same bar (only the grouping of the moment sums changes); the
+    autotuner leaves the state untouched and picks one of its candidates."""
+    path = write_variant("wakes", str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    n = cfg.geti("NrevTot")
+    orc = o.OracleRing(cfg, o.RNG_PHILOX)
+    orc.prepare()
+    ost, _, _ = orc.track(n)
+    g = m.Ring(cfg)
+    g.prepare()
+    best, ms = g.autotune()
+    assert best in (4, 8, 16, 32) and all(v > 0 for v in ms.values())
+    g.set_chunk(items)
+    assert g.chunk() == items
+    gst, _, _ = g.track(n)
+    assert stat_errors(gst, ost).max() <= RTOL
+    hc_o, tr_o = orc.histograms()
+    hc_g, tr_g = g.histograms()
+    assert np.array_equal(hc_o, hc_g) and np.array_equal(tr_o, tr_g)
