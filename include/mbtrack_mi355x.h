@@ -192,6 +192,8 @@ typedef struct mbt_ring mbt_ring;
 #define MBT_RING_NO_INIT_DIST 1 /* leave coordinates zero; caller uploads them */
 #define MBT_RING_NO_GRAPHS 2    /* launch every kernel of every turn eagerly instead of replaying
                                    HIP graphs of whole blocks of turns */
+#define MBT_RING_NO_RW_GEMM 4   /* ensembles: keep the per-bunch resistive-wall sum even where the
+                                   rings share one tap table (otherwise one FP64 MFMA GEMM) */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);

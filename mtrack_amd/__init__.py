@@ -18,6 +18,7 @@ NSTAT = 12
 NAMP = 6
 MBT_RING_NO_INIT_DIST = 1
 MBT_RING_NO_GRAPHS = 2
+MBT_RING_NO_RW_GEMM = 4
 MBT_WS_BUNCH_CURRENTS = 1
 
 _lib = None
@@ -87,8 +88,9 @@ def _check(rc):
         raise MbtError("mbtrack error %d: %s" % (rc, lib().mbt_last_error().decode()))
 
 
-def _flags(init_dist, graphs):
-    return (0 if init_dist else MBT_RING_NO_INIT_DIST) | (0 if graphs else MBT_RING_NO_GRAPHS)
+def _flags(init_dist, graphs, rw_gemm=True):
+    return ((0 if init_dist else MBT_RING_NO_INIT_DIST) | (0 if graphs else MBT_RING_NO_GRAPHS)
+            | (0 if rw_gemm else MBT_RING_NO_RW_GEMM))
 
 
 def dptr(a):
@@ -225,7 +227,7 @@ class Ensemble(Ring):
     Bunch ordinals, stats and ampinv rows run over the rings' bunches in ring
     order; rf rows hold one (phai0, Vrf0) pair per ring."""
 
-    def __init__(self, cfgs, device=0, init_dist=True, graphs=True):
+    def __init__(self, cfgs, device=0, init_dist=True, graphs=True, rw_gemm=True):
         L = lib()
         self.cfgs = list(cfgs)
         self.R = len(self.cfgs)
@@ -237,7 +239,7 @@ class Ensemble(Ring):
         self.first = np.concatenate([[0], np.cumsum(self.nb)])
         arr = (ctypes.c_void_p * self.R)(*[c.ptr.value for c in self.cfgs])
         p = ctypes.c_void_p()
-        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs), ctypes.byref(p)))
+        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs, rw_gemm), ctypes.byref(p)))
         self.ptr = p
 
     def size(self):

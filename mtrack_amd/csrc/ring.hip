@@ -45,6 +45,7 @@
 #endif
 #define STATS_TURNS 256
 #define MIN_CHUNK_ITEMS 4
+#define RWG_MIN_RINGS 8
 #define AUTOTUNE_CANDIDATES 4
 #define MAX_CHUNK_ITEMS 64
 #define LR_PREFETCH 16
@@ -1004,6 +1005,93 @@ __global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, con
   }
 }
 
+// ------------------------------------------------------- RW sum as one GEMM
+
+// Resistive-wall sum of many rings at once (rings of one ensemble with equal h,
+// Nmlt and T0 share the tap weights c[m] = 1 / sqrt(m T0 / h)):
+//   out[kb][col] = sum_a T[kb][a] Y[a][col],  T[kb][a] = c[a - kb] (1 <= a - kb <= M)
+// a Toeplitz tap matrix times one column per (ring, plane) of
+//   Y[a][col] = hist[(-rev h + a) mod n][plane] * Ibunch[a mod h]   (filled buckets)
+// -- the sum of src/transform_weak.c:806-824 for every bunch of every ring; the
+// ring's fNp and RWconst scale the result.  FP64 MFMA (16x16x4), split-K over
+// the four waves of a block.
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+struct RWGemm {
+  int K;        // rows of Y (a = 0 .. K-1), multiple of 16
+  int N;        // columns: 2 per RW ring
+  int M;        // taps (Nmlt + 1) h
+  int h, n;     // buckets, history length (Nmlt + 2) h
+  const int *col_ring;     // [N/2] ring of each column pair
+  const double *c;         // [M + 1] tap weights, c[0] unused
+  double *Y;               // [K][N]
+  const int *bucket_bunch; // [H] global bucket -> bunch (-1 empty)
+};
+
+__global__ __launch_bounds__(256) void k_rwg_y(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                             const TurnState *__restrict__ tsr, const double *rwhist,
+                                             const double *Ibunch, const int *nfFill)
+{
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)Gm.K * Gm.N) return;
+  const int a = (int)(e / Gm.N), col = (int)(e % Gm.N);
+  const int ring = Gm.col_ring[col >> 1], pl = col & 1;
+  const DevParams &P = Pr[ring];
+  double v = 0.0;
+  if (a >= 1 && a < Gm.h + Gm.M) {
+    const int q = a % Gm.h;
+    if (nfFill[P.bucket_off + q]) {
+      const long long rev = tsr[ring].rev;
+      long long idx = (-rev * Gm.h + a) % Gm.n;
+      if (idx < 0) idx += Gm.n;
+      v = rwhist[P.rw_off + 2 * idx + pl] * Ibunch[P.bucket_off + q];
+    }
+  }
+  Gm.Y[e] = v;
+}
+
+// one block (4 waves) per 16 x 16 output tile; wave w covers a quarter of K
+__global__ __launch_bounds__(256) void k_rwg_mfma(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                                const TurnState *__restrict__ tsr, double *pending)
+{
+  __shared__ double acc_s[4][16][17];
+  const int ti = blockIdx.x, tj = blockIdx.y; // kb tile, column tile
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = ti * 16 + (lane & 15); // A row (bucket kb) of this lane
+  const int kq = lane >> 4;            // k within the 4-step
+  const int j = tj * 16 + (lane & 15); // B column of this lane
+  const int ksteps = Gm.K / 4, per = (ksteps + 3) / 4;
+  const int s0 = wv * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
+  mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int st = s0; st < s1; st++) {
+    const int a = st * 4 + kq;
+    const int m = a - i;
+    const double av = (i < Gm.h && m >= 1 && m <= Gm.M) ? Gm.c[m] : 0.0;
+    const double bv = j < Gm.N ? Gm.Y[(size_t)a * Gm.N + j] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  // D[row = 4 (lane / 16) + v][col = lane % 16]
+#pragma unroll
+  for (int v = 0; v < 4; v++) acc_s[wv][4 * (lane >> 4) + v][lane & 15] = acc[v];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    const int r = threadIdx.x >> 4, cc = threadIdx.x & 15;
+    const double sum = ((acc_s[0][r][cc] + acc_s[1][r][cc]) + acc_s[2][r][cc]) + acc_s[3][r][cc];
+    const int kb = ti * 16 + r, col = tj * 16 + cc;
+    if (kb < Gm.h && col < Gm.N) {
+      const int ring = Gm.col_ring[col >> 1], pl = col & 1;
+      const DevParams &P = Pr[ring];
+      const int g = Gm.bucket_bunch[P.bucket_off + kb];
+      if (g >= 0) {
+        const int trig = tsr[ring].m < 50 ? 1 : 0;
+        const bool on = pl == 0 ? P.track_h : P.track_v;
+        pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
+      }
+    }
+  }
+}
+
 // Turn end, one block per ring, for turns without a resistive-wall pass;
 // with fb_here also the RF-feedback step (no long-range resonators anywhere).
 __global__ void k_end(const DevParams *__restrict__ Pr, TurnState *tsr, double *bunch_Ib, double *Ibunch,
@@ -1164,6 +1252,10 @@ struct mbt_ring {
   hipStream_t stream;
   int G, H, Nc;    // bunches, buckets (all rings), slices
   long long Ntot;
+  int rwg_on;        // RW sum as one GEMM over the ensemble's RW rings
+  RWGemm rwg;
+  int *rwg_col_ring, *rwg_bucket_bunch;
+  double *rwg_c, *rwg_Y;
   int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
   int chunk_items;   // macroparticles per lane of one work-group's chunk
   int any_qe, any_dip, any_gen, nrev_mon;
@@ -1346,7 +1438,8 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
                   r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
-                  r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist};
+                  r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
+                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_Y};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
     if (ptrs[i]) (void)hipFree(ptrs[i]);
   if (r->evpool) {
@@ -1468,6 +1561,61 @@ static int set_decomposition(mbt_ring *r, int items)
     drop_graphs(r); // cached graphs hold the old grid
   }
   return rc;
+}
+
+// The GEMM form of the RW sum: used when at least RWG_MIN_RINGS rings have the
+// resistive wall on and all of them share h, Nmlt and T0 (one tap table).
+static int setup_rw_gemm(mbt_ring *r)
+{
+  int nrw = 0, h = 0, Nmlt = 0;
+  double T0 = 0.0;
+  for (int k = 0; k < r->R; k++) {
+    const mbt_config *c = &r->rh[k].cfg;
+    if (!c->EnableRW_long) continue;
+    if (nrw == 0) {
+      h = (int)c->Nharm;
+      Nmlt = c->Nmlt;
+      T0 = c->T0;
+    } else if ((int)c->Nharm != h || c->Nmlt != Nmlt || c->T0 != T0)
+      return MBT_OK; // rings with different taps: the per-bunch sum
+    nrw++;
+  }
+  if (nrw < RWG_MIN_RINGS) return MBT_OK;
+  RWGemm &g = r->rwg;
+  g.h = h;
+  g.M = (Nmlt + 1) * h;
+  g.n = (Nmlt + 2) * h;
+  g.K = ((h + g.M) + 15) / 16 * 16;
+  g.N = 2 * nrw;
+  int *cr = (int *)malloc(sizeof(int) * nrw), *bb = (int *)malloc(sizeof(int) * r->H);
+  double *c = (double *)malloc(sizeof(double) * (g.M + 1));
+  for (int k = 0, i = 0; k < r->R; k++)
+    if (r->rh[k].cfg.EnableRW_long) cr[i++] = k;
+  for (int q = 0; q < r->H; q++) bb[q] = -1;
+  for (int k = 0, gg = 0; k < r->R; k++)
+    for (int b = 0; b < r->rh[k].cfg.Nbunch; b++, gg++) bb[r->Ph[k].bucket_off + r->rh[k].cfg.bucket[b]] = gg;
+  const double deltab = T0 / r->rh[cr[0]].cfg.h;
+  c[0] = 0.0;
+  for (long long m = 1; m <= g.M; m++) c[m] = 1.0 / sqrt(m * deltab);
+  int rc = dalloc(&r->rwg_col_ring, nrw);
+  if (!rc) rc = dalloc(&r->rwg_bucket_bunch, r->H);
+  if (!rc) rc = dalloc(&r->rwg_c, (size_t)g.M + 1);
+  if (!rc) rc = dalloc(&r->rwg_Y, (size_t)g.K * g.N);
+  if (!rc && (hipMemcpy(r->rwg_col_ring, cr, sizeof(int) * nrw, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(r->rwg_bucket_bunch, bb, sizeof(int) * r->H, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(r->rwg_c, c, sizeof(double) * (g.M + 1), hipMemcpyHostToDevice) != hipSuccess ||
+              hipDeviceSynchronize() != hipSuccess))
+    rc = MBT_EDEVICE;
+  free(cr);
+  free(bb);
+  free(c);
+  if (rc) return rc;
+  g.col_ring = r->rwg_col_ring;
+  g.bucket_bunch = r->rwg_bucket_bunch;
+  g.c = r->rwg_c;
+  g.Y = r->rwg_Y;
+  r->rwg_on = 1;
+  return MBT_OK;
 }
 
 static int create_rings(const mbt_config *const *cfgs, int R, int device, int flags, mbt_ring **out)
@@ -1801,6 +1949,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     return MBT_EDEVICE;
   }
   int rcd = set_decomposition(r, CHUNK_ITEMS);
+  if (!rcd && !(flags & MBT_RING_NO_RW_GEMM)) rcd = setup_rw_gemm(r);
   if (rcd) {
     mbt_ring_destroy(r);
     return rcd;
@@ -2006,7 +2155,16 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   if (E) CK(hipEventRecord(E[1], r->stream));
   if ((rc = launch_reduce(r, 2))) return rc;
   if (E) CK(hipEventRecord(E[2], r->stream));
-  if (r->rw_total > 0)
+  if (r->rwg_on) {
+    const long long tot = (long long)r->rwg.K * r->rwg.N;
+    hipLaunchKernelGGL(k_rwg_y, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
+                       r->rwhist, r->Ibunch, r->nfFill);
+    hipLaunchKernelGGL(k_rwg_mfma, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream, r->rwg,
+                       r->P, r->ts, r->pending);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
+                       !lr_launch);
+  } else if (r->rw_total > 0)
     hipLaunchKernelGGL(k_rw, dim3(r->G), dim3(WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
                        r->Ibunch, r->nfFill, r->bucket, r->pending, r->ts, r->ring_done, r->bunch_Ib, !lr_launch);
   else

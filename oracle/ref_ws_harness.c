@@ -204,12 +204,12 @@ int main(int argc, char **argv)
         const int irevmon = rev / track.NrevMon;
         const long nout = (long)(track.NrevTot / track.NrevMon * ebeam.Nbunch) + 2;
         if (wfiles && irevmon * ebeam.Nbunch + kb < nout) {
-        mh->cm[irevmon * ebeam.Nbunch + kb] = b.stats.pos;
-        mh->rms[irevmon * ebeam.Nbunch + kb] = b.stats.pos_sigma;
-        mh->cm_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope;
-        mh->rms_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope_sigma;
-        mh->ampinv[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv;
-        mh->ampinv_cm[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv_cm;
+          mh->cm[irevmon * ebeam.Nbunch + kb] = b.stats.pos;
+          mh->rms[irevmon * ebeam.Nbunch + kb] = b.stats.pos_sigma;
+          mh->cm_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope;
+          mh->rms_slope[irevmon * ebeam.Nbunch + kb] = b.stats.slope_sigma;
+          mh->ampinv[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv;
+          mh->ampinv_cm[irevmon * ebeam.Nbunch + kb] = b.stats.ampinv_cm;
         }
       }
       wb[kb] = b;

@@ -74,3 +74,35 @@ def test_ensemble_matches_oracle(tmp_path):
         orc.close()
     ens.close()
 
+
+
+def test_rw_gemm_equals_per_bunch_sum(tmp_path):
+    """Ten rings sharing one resistive-wall tap table (same h, Nmlt, T0; their
+    own currents, bunch counts and seeds): the ensemble's RW sum runs as one FP64
+    MFMA GEMM.  It must agree with the per-bunch sum (MBT_RING_NO_RW_GEMM) to the
+    rounding of the two summation orders, and each ring with the oracle."""
+    cfgs = []
+    for k in range(10):
+        d = str(tmp_path / ("r%d" % k))
+        path = write_variant("wakes", d, ring={"Iring": 20.0 + 7.0 * k},
+                             distribution={"Np": 400 + 50 * k, "iseed_LON": 11 + k, "iseed_HOR": 31 + k,
+                                           "iseed_VER": 51 + k},
+                             tracking={"NrevTot": TURNS})
+        cfgs.append(m.Config(path, d + "/"))
+    a = m.Ensemble(cfgs)
+    b = m.Ensemble(cfgs, rw_gemm=False)
+    a.prepare()
+    b.prepare()
+    sa, _, _ = a.track(TURNS)
+    sb, _, _ = b.track(TURNS)
+    e = stat_errors(sa, sb)
+    print("gemm vs per-bunch RW: max normalised difference %.3g" % e.max(), flush=True)
+    assert e.max() <= RTOL
+    for k, cfg in enumerate(cfgs[:3]):
+        orc = o.OracleRing(cfg, o.RNG_PHILOX)
+        orc.prepare()
+        ost, _, _ = orc.track(TURNS)
+        assert stat_errors(sa[:, a.first[k]:a.first[k + 1]], ost).max() <= RTOL
+        orc.close()
+    a.close()
+    b.close()
