@@ -90,7 +90,7 @@ def cpu_baseline(np_, turns=2, timeout=240):
                 "sample": sample}
 
 
-def batched(steps, warmup, nrings, seed):
+def batched(steps, warmup, nrings, seed, chunk=0):
     """The batched ensemble: nrings independent rings (bunch and macroparticle
     counts log-uniform, currents and chromaticities uniform, RF beam-loading
     feedback on; mtrack_amd.confgen.wl_ensemble) tracked together, FP64."""
@@ -108,7 +108,11 @@ def batched(steps, warmup, nrings, seed):
                 ens.set_bunch(g, b0)
         ens.prepare()
         R, G, N = ens.size()
-        chunk, tune_ms = ens.autotune()
+        if chunk:
+            ens.set_chunk(chunk)
+            tune_ms = None
+        else:
+            chunk, tune_ms = ens.autotune()
         if warmup > 0:
             ens.track(warmup)
         g0 = time.perf_counter()
@@ -224,6 +228,10 @@ def main():
     ap.add_argument("--batched-rings", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-seed", type=int, default=20240607)
+    ap.add_argument("--batched-chunk", type=int, default=0,
+                    help="work items per lane for the batched line (0: autotune)")
+    ap.add_argument("--no-production", action="store_true",
+                    help="skip the production line (profiling of the other lines only)")
     ap.add_argument("--no-fbii", action="store_true", help="skip the fast beam-ion line")
     ap.add_argument("--fbii-steps", type=int, default=10)
     a = ap.parse_args()
@@ -234,6 +242,14 @@ def main():
         print("bench.py: this build runs on one GPU; the multi-GPU (bunch-sharded) path is not part of it. "
               "Run with --gpus 1.", file=sys.stderr)
         return 2
+    if a.no_production:
+        out = {}
+        if not a.no_batched:
+            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
+        if not a.no_fbii:
+            out["fbii"] = fbii(a.fbii_steps, 2, cpu=False)
+        print(json.dumps(out))
+        return 0
     import mtrack_amd as m
     with tempfile.TemporaryDirectory() as d:
         conf = production_conf(d, a.np_, 20000)
@@ -285,7 +301,7 @@ def main():
     }
     if not a.no_batched:
         try:
-            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed)
+            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
         except Exception as e:  # reported beside the FP64 line, never replaces it
             out["batched"] = {"value": None, "error": str(e)[:300]}
     if not a.no_fbii:

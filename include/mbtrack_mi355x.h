@@ -218,6 +218,12 @@ int mbt_ring_histograms(const mbt_ring *r, double *counts, int *trash);
  * dipoleH (the columns of potentials_bunch_<kb>.dat, src/transform_weak.c:735-768);
  * extra[3]: phase and voltage before the next turn's RF-feedback solve, scan value. */
 int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, double *extra);
+/* RF state of ring `ring` (0 for a single ring) after the last tracked turn:
+ * out[3] = phai0, Vrf0 [MV] the last turn's map used (before a turn has run:
+ * the configured ones) and whether the RF feedback is still on -- the
+ * worker's ring.phai0, ring.Vrf0 and ring.has_rf_feedback printed at the end of
+ * the run (src/worker_weakweak.c:446-448). */
+int mbt_ring_rf_state(const mbt_ring *r, int ring, double *out);
 /* RW dipole history [(Nmlt+2)*h][2] in the reference's cyclic layout
  * (src/worker_weakweak.c:203-219). */
 int mbt_ring_rw_history(const mbt_ring *r, double *out);

@@ -67,6 +67,7 @@ def lib():
         L.mbt_ring_set_chunk.argtypes = [vp, ip]
         L.mbt_ring_get_chunk.argtypes = [vp]
         L.mbt_ring_autotune.argtypes = [vp, ctypes.POINTER(ip), dp]
+        L.mbt_ring_rf_state.argtypes = [vp, ip, dp]
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
         L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
@@ -212,6 +213,12 @@ class Ring:
 
     def chunk(self):
         return lib().mbt_ring_get_chunk(self.ptr)
+
+    def rf_state(self, ring=0):
+        """(phai0, Vrf0 [MV], feedback on) of ring `ring` after the last turn."""
+        out = np.zeros(3)
+        _check(lib().mbt_ring_rf_state(self.ptr, ring, dptr(out)))
+        return out[0], out[1], bool(out[2])
 
     def autotune(self):
         """Chooses the per-work-group chunk by timing; returns (items, {items: ms})."""

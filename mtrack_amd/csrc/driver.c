@@ -387,12 +387,17 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
   {
     int *trash = malloc(sizeof(int) * 2 * Nb);
     double *cnt = malloc(sizeof(double) * (size_t)Nb * (Nc > 0 ? Nc : 1));
-    if (Nc > 0 && mbt_ring_histograms(r, cnt, trash) == MBT_OK)
-      for (int k = 0; k < Nb; k++)
-        if (trash[2 * k] != 0 || trash[2 * k + 1] != 0)
-          printf("\n WARNING - Binning for selffields bunch #%d:\n %d particles below first bin,    %d particles above "
-                 "last bin\n",
-                 c->bucket[k], trash[2 * k], trash[2 * k + 1]);
+    const int have_hist = Nc > 0 && mbt_ring_histograms(r, cnt, trash) == MBT_OK;
+    double rf[3] = {0.0, 0.0, 0.0};
+    const int have_rf = mbt_ring_rf_state(r, 0, rf) == MBT_OK && rf[2] != 0.0;
+    /* each worker's last lines, in bunch order (src/worker_weakweak.c:441-448) */
+    for (int k = 0; k < Nb; k++) {
+      if (have_hist && (trash[2 * k] != 0 || trash[2 * k + 1] != 0))
+        printf("\n WARNING - Binning for selffields bunch #%d:\n %d particles below first bin,    %d particles above "
+               "last bin\n",
+               c->bucket[k], trash[2 * k], trash[2 * k + 1]);
+      if (have_rf && is_out(c, c->bucket[k])) printf("\nFinal RF voltage and phase: %.7f MV, %.5f\n", rf[1], rf[0]);
+    }
     free(trash);
     free(cnt);
   }

@@ -49,6 +49,9 @@
 #define AUTOTUNE_CANDIDATES 4
 #define MAX_CHUNK_ITEMS 64
 #define LR_PREFETCH 16
+// every bunch's coordinates start on a 256-byte boundary (32 doubles), so a
+// chunk's loads and stores cover whole cache lines whatever Np is
+#define BUNCH_ALIGN 32
 #ifndef COORD_NONTEMPORAL
 #define COORD_NONTEMPORAL 1
 #endif
@@ -1071,9 +1074,9 @@ __global__ __launch_bounds__(256) void k_rwg_mfma(RWGemm Gm, const DevParams *__
     const double bv = j < Gm.N ? Gm.Y[(size_t)a * Gm.N + j] : 0.0;
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
   }
-  // D[row = 4 (lane / 16) + v][col = lane % 16]
+  // f64 16x16x4 result layout: element v of lane l is D[row = l / 16 + 4 v][col = l % 16]
 #pragma unroll
-  for (int v = 0; v < 4; v++) acc_s[wv][4 * (lane >> 4) + v][lane & 15] = acc[v];
+  for (int v = 0; v < 4; v++) acc_s[wv][(lane >> 4) + 4 * v][lane & 15] = acc[v];
   __syncthreads();
   if (threadIdx.x < 256) {
     const int r = threadIdx.x >> 4, cc = threadIdx.x & 15;
@@ -1251,7 +1254,8 @@ struct mbt_ring {
   int device;
   hipStream_t stream;
   int G, H, Nc;    // bunches, buckets (all rings), slices
-  long long Ntot;
+  long long Ntot;   // macroparticles
+  long long Nalloc; // coordinate slots: bunches padded to BUNCH_ALIGN
   int rwg_on;        // RW sum as one GEMM over the ensemble's RW rings
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
@@ -1498,6 +1502,8 @@ static int check_cfg(const mbt_config *c)
   return MBT_OK;
 }
 
+static long long padded_np(long long Np) { return (Np + BUNCH_ALIGN - 1) / BUNCH_ALIGN * BUNCH_ALIGN; }
+
 static int wg_count(const mbt_ring *r, int items)
 {
   const long long CH = (long long)WG * items;
@@ -1538,7 +1544,7 @@ static int set_decomposition(mbt_ring *r, int items)
         we[w] = pos + ((kk + 1) * CH < Np ? (kk + 1) * CH : Np);
         wb[w] = g;
       }
-      pos += Np;
+      pos += padded_np(Np);
     }
   }
   b0[G] = w;
@@ -1654,7 +1660,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   // ---- per-ring parameters, Green functions, offsets
   double *Gf = (double *)calloc((size_t)R * 3 * NcA, sizeof(double));
   int G = 0, H = 0;
-  long long Ntot = 0, rw_total = 0, st_total = 0, fbh_total = 0;
+  long long Ntot = 0, Nalloc = 0, rw_total = 0, st_total = 0, fbh_total = 0;
   for (int k = 0; k < R; k++) {
     RingHost *hh = &r->rh[k];
     const mbt_config *c = &hh->cfg;
@@ -1676,6 +1682,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     G += c->Nbunch;
     H += (int)c->Nharm;
     Ntot += (long long)c->Np * c->Nbunch;
+    Nalloc += padded_np(c->Np) * c->Nbunch;
     if (c->EnableRW_long) {
       rw_total += 2LL * (c->Nmlt + 2) * c->Nharm;
       st_total += (long long)(c->Nmlt + 1) * c->Nharm + 1;
@@ -1692,6 +1699,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   r->G = G;
   r->H = H;
   r->Ntot = Ntot;
+  r->Nalloc = Nalloc;
   r->rw_total = rw_total;
   r->st_total = st_total;
   r->fbh_total = fbh_total;
@@ -1720,16 +1728,16 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
         bk[g] = c->bucket[bb];
         bIb[g] = c->bunch_Ib[bb];
         bfirst[g] = pos;
-        pos += Np;
+        pos += padded_np(Np);
       }
     }
   }
-  DA(r->C.tau, Ntot);
-  DA(r->C.del, Ntot);
-  DA(r->C.x, Ntot);
-  DA(r->C.xp, Ntot);
-  DA(r->C.z, Ntot);
-  DA(r->C.zp, Ntot);
+  DA(r->C.tau, Nalloc);
+  DA(r->C.del, Nalloc);
+  DA(r->C.x, Nalloc);
+  DA(r->C.xp, Nalloc);
+  DA(r->C.z, Nalloc);
+  DA(r->C.zp, Nalloc);
   DA(r->wg_first, nwg);
   DA(r->wg_end, nwg);
   DA(r->wg_bunch, nwg);
@@ -1976,10 +1984,10 @@ static long long bunch_first_host(const mbt_ring *r, int g, int *np)
     const mbt_config *c = &r->rh[k].cfg;
     if (g < c->Nbunch) {
       *np = c->Np;
-      return pos + (long long)g * c->Np;
+      return pos + (long long)g * padded_np(c->Np);
     }
     g -= c->Nbunch;
-    pos += (long long)c->Nbunch * c->Np;
+    pos += (long long)c->Nbunch * padded_np(c->Np);
   }
   *np = 0;
   return -1;
@@ -2331,6 +2339,21 @@ extern "C" int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, doubl
     rfprev[1] = ts.Vrf0_prev;
     rfprev[2] = ts.scan_val;
   }
+  return MBT_OK;
+}
+
+extern "C" int mbt_ring_rf_state(const mbt_ring *r, int ring, double *out)
+{
+  if (ring < 0 || ring >= r->R || !out) return MBT_EINVAL;
+  CK(hipSetDevice(r->device));
+  CK(hipStreamSynchronize(r->stream));
+  TurnState ts;
+  CK(hipMemcpy(&ts, r->ts + ring, sizeof ts, hipMemcpyDeviceToHost));
+  // every turn ends with the feedback step, which first saves the phase and
+  // voltage the turn's map used
+  out[0] = ts.phai0_prev;
+  out[1] = ts.Vrf0_prev;
+  out[2] = ts.has_fb;
   return MBT_OK;
 }
 

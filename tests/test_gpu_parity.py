@@ -67,6 +67,10 @@ def test_turns_match_oracle(tmp_path, name):
     assert ea.max() <= RTOL
     er = np.abs(grf - orf) / np.abs(orf)
     assert er.max() <= RTOL
+    # the end-of-run RF state is the one the last turn's map used
+    ph, vrf, fb_on = g.rf_state()
+    assert (ph, vrf) == (grf[-1, 0], grf[-1, 1])
+    assert not fb_on or cfg.geti("has_rf_feedback")
     if cfg.geti("Ncell") > 0:
         hc_o, tr_o = orc.histograms()
         hc_g, tr_g = g.histograms()
