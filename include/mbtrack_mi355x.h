@@ -241,7 +241,9 @@ long mbt_ring_turn(const mbt_ring *r);
  * only at the rounding level (grouping of the moment sums). */
 int mbt_ring_set_chunk(mbt_ring *r, int items);
 int mbt_ring_get_chunk(const mbt_ring *r);
-/* Times the statistics pass for items = 4, 8, 16, 32 (ms[4], may be NULL),
+/* Times the ring's own per-macroparticle turn kernel for items = 4, 8, 16, 32
+ * (ms[4], may be NULL) -- the map, kicks, histogram and moment sums computed,
+ * the coordinates written back unchanged, so the ring's state is untouched --
  * keeps the fastest and returns it in *best_items.  Call after mbt_ring_prepare. */
 int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
 
@@ -274,6 +276,11 @@ typedef struct mbt_ws mbt_ws;
  * src/manager_weakweak.c:58-80) instead of the zeros the reference leaves in
  * ring.Ibunch in this mode (which make the ions chargeless) */
 #define MBT_WS_BUNCH_CURRENTS 1
+/* flags: run the ion train as the reference's relay, one launch per bunch in
+ * train order, instead of one lane per ion over all later bunches (the default;
+ * the relay is also taken when the default's per-bunch partials would exceed
+ * 8 GiB).  Both give the same sums up to their summation order. */
+#define MBT_WS_SERIAL_TRAIN 2
 int mbt_ws_create(const mbt_config *cfg, int device, int flags, mbt_ws **out);
 void mbt_ws_destroy(mbt_ws *w);
 /* nturns turns; per turn (any output may be NULL):

@@ -20,6 +20,7 @@ MBT_RING_NO_INIT_DIST = 1
 MBT_RING_NO_GRAPHS = 2
 MBT_RING_NO_RW_GEMM = 4
 MBT_WS_BUNCH_CURRENTS = 1
+MBT_WS_SERIAL_TRAIN = 2
 
 _lib = None
 
@@ -294,14 +295,15 @@ class WeakStrong:
     """Weak-strong bunches with the fast beam-ion interaction on one GPU
     (mbt_ws_* C ABI)."""
 
-    def __init__(self, cfg, device=0, bunch_currents=True):
+    def __init__(self, cfg, device=0, bunch_currents=True, serial_train=False):
         L = lib()
         self.cfg = cfg
         self.Nb = cfg.geti("Nbunch")
         self.h = cfg.geti("Nharm")
         self.S = 2 * cfg.geti("Nslc") + 1
         p = ctypes.c_void_p()
-        _check(L.mbt_ws_create(cfg.ptr, device, MBT_WS_BUNCH_CURRENTS if bunch_currents else 0, ctypes.byref(p)))
+        flags = (MBT_WS_BUNCH_CURRENTS if bunch_currents else 0) | (MBT_WS_SERIAL_TRAIN if serial_train else 0)
+        _check(L.mbt_ws_create(cfg.ptr, device, flags, ctypes.byref(p)))
         self.ptr = p
 
     def close(self):

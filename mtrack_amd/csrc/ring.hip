@@ -7,10 +7,13 @@
 // rings of uneven bunch and macroparticle counts share each kernel.  One turn
 // is four kernels on one stream:
 //
-//   k_lr      long-range resonator phasor recurrence over all buckets and the
-//             RF-feedback solve (construct_wake_phasor src/transform_weak.c:961-1048,
-//             src/worker_weakweak.c:268-278): sequential by nature, one lane per
-//             resonator, bit-identical arithmetic to the reference;
+//   k_lr_par  long-range resonator phasor recurrence over all buckets and the
+//             RF-feedback samples (construct_wake_phasor src/transform_weak.c:961-1048):
+//             linear, so one block per resonator with the buckets in parallel and
+//             one short chain over bucket start phasors; k_lr keeps the one-lane
+//             recurrence (bit-identical arithmetic to the reference) for
+//             wake_phasor_init and very large h; k_fb_solve (or the turn end)
+//             runs the RF-feedback solve (src/worker_weakweak.c:268-278);
 //   k_turn    one pass over every macroparticle: pending RW kick, selffield
 //             kick (src/transform_weak.c:697-731), one-turn map
 //             (src/transform_weak.c:21-192), shifted moment sums and the next
@@ -49,6 +52,9 @@
 #define AUTOTUNE_CANDIDATES 4
 #define MAX_CHUNK_ITEMS 64
 #define LR_PREFETCH 16
+// dynamic LDS of the bucket-parallel LR recurrence (6 doubles per bucket): up
+// to h = 1365; larger rings use the one-lane recurrence
+#define LR_PAR_LDS_MAX (64 * 1024)
 // every bunch's coordinates start on a 256-byte boundary (32 doubles), so a
 // chunk's loads and stores cover whole cache lines whatever Np is
 #define BUNCH_ALIGN 32
@@ -278,7 +284,11 @@ __device__ __forceinline__ double fx_to_double(unsigned long long lo, unsigned l
 
 // ------------------------------------------------------------------ k_turn
 
-enum { MODE_TURN = 0, MODE_STATS = 1, MODE_FINAL = 2 };
+// MODE_TUNE: the turn's full work for timing work decompositions -- the map's
+// results feed the moment sums and the histogram, the coordinates are written
+// back unchanged (every store and atomic target is scratch the next pass
+// overwrites or the caller clears)
+enum { MODE_TURN = 0, MODE_STATS = 1, MODE_FINAL = 2, MODE_TUNE = 3 };
 
 struct HistPtr {
   unsigned int *cnt;        // [Nb*Nc]
@@ -430,7 +440,7 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
     }
   for (int i = tid; i < Nc; i += WG) {
     lcnt[i] = 0;
-    if (MODE == MODE_TURN) {
+    if (MODE == MODE_TURN || MODE == MODE_TUNE) {
       if (P.kickL) kL[i] = sfL[(size_t)b * Nc + i] + lrL[i];
       if (P.kickV) kV[i] = sfV[(size_t)b * Nc + i] + lrV[i];
       if (P.kickH) kH[i] = sfH[(size_t)b * Nc + i] + lrH[i];
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
            zp = LDC(pzp + i);
     if (rw_h) xp += rwx;
     if (rw_v) zp += rwz;
-    if (MODE == MODE_TURN) {
+    if (MODE == MODE_TURN || MODE == MODE_TUNE) {
       if (P.sf_active) {
         int bn = bin_fast(tau, bk);
         bn = bn < 0 ? 0 : (bn >= Nc ? Nc - 1 : bn);
@@ -540,7 +550,14 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
         }
       }
     }
-    if (MODE != MODE_STATS) {
+    if (MODE == MODE_TUNE) {
+      STC(LDC(pt + i), pt + i);
+      STC(LDC(pd + i), pd + i);
+      STC(LDC(px + i), px + i);
+      STC(LDC(pxp + i), pxp + i);
+      STC(LDC(pz + i), pz + i);
+      STC(LDC(pzp + i), pzp + i);
+    } else if (MODE != MODE_STATS) {
       STC(tau, pt + i);
       STC(del, pd + i);
       STC(x, px + i);
@@ -1196,6 +1213,125 @@ __global__ void k_lr(int nres, const LRConst *lc, const DevParams *__restrict__ 
   if (fb) ts->fb_count = cnt;
 }
 
+// The per-turn recurrence of one resonator with its buckets in parallel (one
+// block per resonator).  The recurrence is linear: a filled bucket maps its
+// start phasor V to G (p^Nc V + L), with L the bucket's own sum started from
+// zero, an empty one to E V.  Phase A: every bucket's L (one lane per bucket,
+// the recurrence's own operations); phase B: one lane chains the h start
+// phasors; phase C: every bucket replays its slices from its start phasor
+// (again the recurrence's own operations) and writes the slice terms and the
+// RF-feedback samples.  Only the start phasors are rounded differently from
+// the sequential loop (by the closed form of phase B).  Dynamic LDS: 6 h doubles.
+__global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevParams *__restrict__ Pr,
+                                                const double *fnp_ring, long long plane_stride, const double *Ibunch,
+                                                const int *nfFill, int Nc, double *phasor_end, double *lrc,
+                                                double *fb_hist, TurnState *tsr)
+{
+  extern __shared__ double lsm[];
+  __shared__ double Pn[2];
+  const int r = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
+  const LRConst L = lc[r];
+  const DevParams &P = Pr[L.ring];
+  TurnState *ts = tsr + L.ring;
+  const int h = L.h;
+  const double *fnp = fnp_ring + (size_t)L.plane * plane_stride + (size_t)L.bucket_off * Nc;
+  const double *Ib = Ibunch + L.bucket_off;
+  const int *fill = nfFill + L.bucket_off;
+  double *__restrict__ out = lrc + L.lrc_off;
+  double *__restrict__ fbh = fb_hist + P.fbh_off;
+  const int fb_len = P.fb_len;
+  const bool fb = L.fb && ts->has_fb;
+  const long long cnt0 = ts->fb_count;
+  double *Lr = lsm, *Li = Lr + h, *Sr = Li + h, *Si = Sr + h, *Ar = Si + h, *Ai = Ar + h;
+  // ---- phase A
+  for (int i = tid; i < h; i += nth) {
+    double V0 = 0.0, V1 = 0.0;
+    if (fill[i] == 1) {
+      const double Ii = Ib[i];
+      const double *f = fnp + (size_t)i * Nc;
+      for (int j = 0; j < Nc; j++) {
+        const double n0 = (V0 * L.pr0 - V1 * L.pr1) - Ii * f[j] * L.fac;
+        const double n1 = (V0 * L.pr1 + V1 * L.pr0);
+        V0 = n0;
+        V1 = n1;
+      }
+    }
+    Lr[i] = V0;
+    Li[i] = V1;
+  }
+  if (tid == 0) {
+    double a = 1.0, b = 0.0;
+    for (int j = 0; j < Nc; j++) {
+      const double na = a * L.pr0 - b * L.pr1, nb = a * L.pr1 + b * L.pr0;
+      a = na;
+      b = nb;
+    }
+    Pn[0] = a;
+    Pn[1] = b;
+  }
+  __syncthreads();
+  // ---- phase B, in the recurrence's bucket order (last bucket first)
+  if (tid == 0) {
+    double V0 = phasor_end[2 * r], V1 = phasor_end[2 * r + 1];
+    const double g0 = L.ge * L.gc, g1 = L.ge * L.gs, e0 = L.ee * L.ec, e1 = L.ee * L.es;
+    for (int m = 0; m < h; m++) {
+      const int i = h - m - 1;
+      Sr[i] = V0;
+      Si[i] = V1;
+      if (fill[i] == 1) {
+        const double w0 = (V0 * Pn[0] - V1 * Pn[1]) + Lr[i], w1 = (V0 * Pn[1] + V1 * Pn[0]) + Li[i];
+        V0 = w0 * g0 - w1 * g1;
+        V1 = w0 * g1 + w1 * g0;
+      } else {
+        const double n0 = V0 * e0 - V1 * e1, n1 = V0 * e1 + V1 * e0;
+        V0 = n0;
+        V1 = n1;
+      }
+      Ar[i] = V0;
+      Ai[i] = V1;
+    }
+    phasor_end[2 * r] = V0;
+    phasor_end[2 * r + 1] = V1;
+  }
+  __syncthreads();
+  // ---- phase C; of the 2 h feedback samples only the last fb_len survive in
+  // the sample ring, so only those are written (no two lanes share a slot)
+  const long long keep_from = cnt0 + 2LL * h - fb_len;
+  for (int i = tid; i < h; i += nth) {
+    if (fill[i] == 1) {
+      double V0 = Sr[i], V1 = Si[i];
+      const double Ii = Ib[i];
+      const double *f = fnp + (size_t)i * Nc;
+      double *o = out + (size_t)i * Nc;
+      for (int j = 0; j < Nc; j++) {
+        const double fj = f[j];
+        const double n0 = (V0 * L.pr0 - V1 * L.pr1) - Ii * fj * L.fac;
+        const double n1 = (V0 * L.pr1 + V1 * L.pr0);
+        o[j] = n0 + 0.5 * Ii * fj * L.fac;
+        V0 = n0;
+        V1 = n1;
+      }
+    }
+    if (fb) {
+      const long long c = cnt0 + 2LL * (h - i - 1);
+      if (c >= keep_from) {
+        const double vr = Sr[i] * L.pb0 - Si[i] * L.pb1, vi = Sr[i] * L.pb1 + Si[i] * L.pb0;
+        const unsigned key = ((unsigned)c % (unsigned)fb_len) * 2;
+        fbh[key] = sqrt(vr * vr + vi * vi);
+        fbh[key + 1] = atan2(vr, vi);
+      }
+      if (c + 1 >= keep_from) {
+        const double vr = Ar[i] * L.pa0 - Ai[i] * L.pa1, vi = Ar[i] * L.pa1 + Ai[i] * L.pa0;
+        const unsigned key = ((unsigned)(c + 1) % (unsigned)fb_len) * 2;
+        fbh[key] = sqrt(vr * vr + vi * vi);
+        fbh[key + 1] = atan2(vr, vi);
+      }
+    }
+  }
+  __syncthreads(); // every lane has read the sample counter
+  if (tid == 0 && fb) ts->fb_count = cnt0 + 2LL * h;
+}
+
 // lr_wake[plane][global bucket][j] = sum over the ring's resonators of that
 // plane, in order, of their terms
 __global__ void k_lr_sum(const int *lr_first, const LRConst *lc, const double *lrc, const int *nfFill,
@@ -1284,6 +1420,7 @@ struct mbt_ring {
   LRConst *lc;
   int *lr_first; // [R+1] first resonator of each ring in lc
   int nres;
+  int lr_hmax; // largest h among the resonators' rings
   double *phasor_end, *fb_hist;
   long long rw_total, st_total, fbh_total;
   size_t lds_turn, lds_red;
@@ -1803,6 +1940,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
           L.plane = p;
           L.l = l;
           L.h = (int)cf->Nharm;
+          if (L.h > r->lr_hmax) r->lr_hmax = L.h;
           L.bucket_off = r->Ph[k].bucket_off;
           L.lrc_off = lrc_total;
           L.init_turns = (int)(cf->Nbumax / cf->h);
@@ -2036,6 +2174,11 @@ static const turn_kernel_t g_turn_kernels[16] = {
     TK(true, true, false, false),   TK(true, true, false, true),   TK(true, true, true, false),
     TK(true, true, true, true)};
 #undef TK
+#define TT(Q, D, G) k_turn<MODE_TUNE, Q, D, G, false>
+static const turn_kernel_t g_tune_kernels[8] = {TT(false, false, false), TT(false, false, true), TT(false, true, false),
+                                                TT(false, true, true),   TT(true, false, false), TT(true, false, true),
+                                                TT(true, true, false),   TT(true, true, true)};
+#undef TT
 
 static int launch_turn_pass(mbt_ring *r, int mode)
 {
@@ -2066,8 +2209,13 @@ static int launch_lr(mbt_ring *r, int init)
 {
   if (r->nres == 0) return MBT_OK;
   const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
-  hipLaunchKernelGGL(k_lr, dim3((r->nres + 63) / 64), dim3(64), 0, r->stream, r->nres, r->lc, r->P, r->fnp_ring,
-                     stride, r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
+  const size_t lds = sizeof(double) * 6 * (size_t)r->lr_hmax;
+  if (!init && lds <= LR_PAR_LDS_MAX)
+    hipLaunchKernelGGL(k_lr_par, dim3(r->nres), dim3(256), lds, r->stream, r->lc, r->P, r->fnp_ring, stride,
+                       r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts);
+  else
+    hipLaunchKernelGGL(k_lr, dim3((r->nres + 63) / 64), dim3(64), 0, r->stream, r->nres, r->lc, r->P, r->fnp_ring,
+                       stride, r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
   CK(hipGetLastError());
   if (!init) {
     long long tot = 3LL * r->H * r->Nc;
@@ -2419,8 +2567,8 @@ extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const turn_kernel_t k = r->any_dip ? k_turn<MODE_STATS, false, true, true, false>
-                                     : k_turn<MODE_STATS, false, false, false, false>;
+  // the turn's own variant, run as a timing pass (coordinates unchanged)
+  const turn_kernel_t k = g_tune_kernels[(r->any_qe ? 4 : 0) + (r->any_dip ? 2 : 0) + (r->any_gen ? 1 : 0)];
   const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
   int best = r->chunk_items, rc = MBT_OK;
   double best_ms = 1e300;

@@ -17,13 +17,19 @@
 //                (transformSbunch_optic, :194-253) and the resistive-wall kick of
 //                every other bucket's centroid history (transformSbunch_longrange,
 //                :69-192), one lane per source bucket;
-//   k_ws_train   bunch by bunch in train order (the reference's manager relay):
-//                the accumulated ion cloud drifts to the bunch, passes its slices
-//                (bilinear field interpolation, src/fbi_weakstrong.c:454-580),
-//                kicks the slice centroids with the summed ion field, and the
-//                bunch's own ions join the cloud (fbi_weakstrong_train :301-451);
-//                one lane per ion, per-slice block sums, the last block of each
-//                launch reduces them in a fixed order;
+//   train        the reference's manager relay (fbi_weakstrong_train
+//                :301-451): the accumulated ion cloud drifts to each bunch in
+//                train order, passes its slices (bilinear field interpolation,
+//                src/fbi_weakstrong.c:454-580), kicks the slice centroids with
+//                the summed ion field, and the bunch's own ions join the cloud.
+//                An ion's path depends only on the field tables phase 1 made
+//                and on the drift times, never on the kicks the train gives the
+//                electrons, so the whole relay is one launch with one lane per
+//                ion over every later bunch (k_ws_ions: per-slice wave sums,
+//                block partials per bunch) and one per-bunch reduction that
+//                applies the kicks (k_ws_kick); k_ws_cloud_reset + k_ws_train
+//                keep the bunch-by-bunch relay (one launch per bunch) for
+//                MBT_WS_SERIAL_TRAIN and for trains whose partials would not fit;
 //   k_ws_end     slice-averaged vertical centroid of every bucket.
 //
 // Everything but the ion-field sums, sin and the Faddeeva function keeps the
@@ -45,6 +51,8 @@
 #define WS_MSG 5000   // doubles of the history message (src/manager_weakstrong.c:170-173)
 #define WS_T 256      // turns of outputs per device buffer block
 #define WS_WG 256
+// device memory the ion-parallel train may use for its per-bunch partials
+#define WS_TRAIN_PARTIAL_MAX (8LL << 30)
 
 #define K_C 2.997925e+08
 #define K_EPS0 8.854187817620e-12
@@ -88,6 +96,10 @@ struct WSD {
   double *part;     // [blocks][S][2]
   double *psum;     // [blocks][3]
   unsigned *done;
+  const double *dtime;    // [G] drift time from the previous bunch
+  double *tpart;          // ion-parallel train: per bunch ib, [blocks with an ion ahead of ib][2 S + 3]
+  const long long *toff;  // [G] offset of bunch ib's partials in tpart
+  int nblk_ions;          // blocks of k_ws_ions
   double *cm, *mon, *ion, *zcm; // output blocks [WS_T][G][4|18|3], [WS_T][h]
 };
 
@@ -507,6 +519,134 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
   }
 }
 
+// Ion-parallel train: lane q carries ion q = j * Nions + p (ion p made by bunch
+// j, its state after the creator's later slices) through bunches j+1 .. G-1,
+// each reached after its drift time.  Per bunch ib the block writes the 2 S
+// slice sums of Nri * E and the cloud sums Nri x, Nri y, Nri after the bunch,
+// each the fixed-order sum of its four waves' sums.
+__global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
+{
+  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  __shared__ double wred[2 * WS_S + 3][WS_WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25, NQ = 2 * S + 3;
+  const size_t GP = (size_t)P.G * npts;
+  const long long nI = (long long)(P.G - 1) * npts;
+  const long long q0 = (long long)blockIdx.x * WS_WG, q = q0 + tid;
+  const int jmin = (int)(q0 / npts);
+  const bool valid = q < nI;
+  const int j = valid ? (int)(q / npts) : P.G;
+  double Nri = 0.0, x = 0.0, xd = 0.0, y = 0.0, yd = 0.0;
+  if (valid) {
+    Nri = D.li[q];
+    x = D.li[GP + q];
+    xd = D.li[2 * GP + q];
+    y = D.li[3 * GP + q];
+    yd = D.li[4 * GP + q];
+  }
+  for (int ib = jmin + 1; ib < P.G; ib++) {
+    __syncthreads(); // the previous bunch's table and sums have been read
+    for (int k = tid; k < npts; k += WS_WG) {
+      ex_s[k] = D.Ex[(size_t)ib * npts + k];
+      ey_s[k] = D.Ey[(size_t)ib * npts + k];
+    }
+    __syncthreads();
+    const bool act = j < ib;
+    if (act) {
+      const double dt = D.dtime[ib];
+      x = x + dt * xd;
+      y = y + dt * yd;
+    }
+    for (int c = 0; c < S; c++) {
+      double sx = 0.0, sy = 0.0;
+      if (act) {
+        double Ex, Ey;
+        ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+        const double ax = P.factF * Ex;
+        xd = xd + P.deltT * ax;
+        x = x + P.deltT * xd;
+        const double ay = P.factF * Ey;
+        yd = yd + P.deltT * ay;
+        y = y + P.deltT * yd;
+        sx = Nri * Ex;
+        sy = Nri * Ey;
+      }
+      sx = wsum64(sx);
+      sy = wsum64(sy);
+      if (lane == 0) {
+        wred[2 * c][wv] = sx;
+        wred[2 * c + 1][wv] = sy;
+      }
+    }
+    double s0 = act ? Nri * x : 0.0, s1 = act ? Nri * y : 0.0, s2 = act ? Nri : 0.0;
+    s0 = wsum64(s0);
+    s1 = wsum64(s1);
+    s2 = wsum64(s2);
+    if (lane == 0) {
+      wred[2 * S][wv] = s0;
+      wred[2 * S + 1][wv] = s1;
+      wred[2 * S + 2][wv] = s2;
+    }
+    __syncthreads();
+    for (int i = tid; i < NQ; i += WS_WG) {
+      double v = 0.0;
+      for (int w = 0; w < WS_WG / 64; w++) v += wred[i][w];
+      D.tpart[D.toff[ib] + (long long)blockIdx.x * NQ + i] = v;
+    }
+  }
+}
+
+// Per bunch: the blocks' partials summed in a fixed order (wave i of the
+// quantities, lanes over blocks), the slice kicks of the ion field, and the
+// cloud sums after the bunch with its own ions appended.
+__global__ __launch_bounds__(WS_WG) void k_ws_kick(WSP P, WSD D, int slot)
+{
+  __shared__ double tot[2 * WS_S + 3];
+  __shared__ double red3[WS_WG / 64][3];
+  const int ib = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25;
+  const int NQ = 2 * S + 3;
+  const size_t GP = (size_t)P.G * npts;
+  long long nb = ((long long)ib * npts + WS_WG - 1) / WS_WG;
+  if (nb > D.nblk_ions) nb = D.nblk_ions;
+  for (int i = wv; i < NQ; i += WS_WG / 64) {
+    double v = 0.0;
+    for (long long k = lane; k < nb; k += 64) v += D.tpart[D.toff[ib] + k * NQ + i];
+    v = wsum64(v);
+    if (lane == 0) tot[i] = v;
+  }
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  for (int p = tid; p < npts; p += WS_WG) {
+    const size_t o = (size_t)ib * npts + p;
+    t0 += D.li[o] * D.li[GP + o];
+    t1 += D.li[o] * D.li[3 * GP + o];
+    t2 += D.li[o];
+  }
+  t0 = wsum64(t0);
+  t1 = wsum64(t1);
+  t2 = wsum64(t2);
+  if (lane == 0) {
+    red3[wv][0] = t0;
+    red3[wv][1] = t1;
+    red3[wv][2] = t2;
+  }
+  __syncthreads();
+  if (ib > 0) {
+    const double nel = D.nelec[ib];
+    double *xp = D.sl + (size_t)ib * 6 * S + 3 * S, *zp = xp + 2 * S;
+    for (int c = tid; c < S; c += WS_WG) {
+      const double dQ = nel * D.npop[c] * K_E;
+      const double fjp = dQ > 0.0 ? P.factjp0 / dQ : 0.0;
+      const double sx = fjp * tot[2 * c], sy = fjp * tot[2 * c + 1];
+      xp[c] = xp[c] + P.CdeltT * sx;
+      zp[c] = zp[c] + P.CdeltT * sy;
+    }
+  }
+  if (tid < 3) {
+    double v = ib > 0 ? tot[2 * S + tid] : 0.0;
+    for (int w = 0; w < WS_WG / 64; w++) v += red3[w][tid];
+    D.ion[((size_t)slot * P.G + ib) * 3 + tid] = v;
+  }
+}
+
 // slice-averaged vertical centroid of every bucket (writeoutVCMmotionsPM2,
 // src/fbi_weakstrong.c:688-737)
 __global__ void k_ws_end(WSP P, WSD D, const int *bunch_of_bucket, int slot)
@@ -539,6 +679,9 @@ struct mbt_ws {
   double *nelec_h;
   long long rev;
   int maxblk;
+  int serial;          // bunch-by-bunch relay instead of the ion-parallel train
+  double *d_dtime;
+  long long *d_toff;
 };
 
 extern "C" void mbt_ws_destroy(mbt_ws *w)
@@ -547,7 +690,7 @@ extern "C" void mbt_ws_destroy(mbt_ws *w)
   (void)hipSetDevice(w->device);
   void *p[] = {w->D.sl,  w->D.hist, w->D.Ex,   w->D.Ey,  w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
                w->D.done, w->D.cm,  w->D.mon,  w->D.ion, w->D.zcm,    w->d_bucket, w->d_fill,    w->d_npop,
-               w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion};
+               w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff};
   for (size_t i = 0; i < sizeof p / sizeof p[0]; i++)
     if (p[i]) (void)hipFree(p[i]);
   if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -712,11 +855,39 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
     WA(w->D.Ex, (size_t)G * npts);
     WA(w->D.Ey, (size_t)G * npts);
     WA(w->D.li, (size_t)5 * G * npts);
-    w->D.cap = (long long)G * npts;
-    WA(w->D.cl, (size_t)5 * w->D.cap);
-    w->maxblk = (int)((w->D.cap + WS_WG - 1) / WS_WG);
-    WA(w->D.part, (size_t)w->maxblk * S * 2);
-    WA(w->D.psum, (size_t)w->maxblk * 3);
+    // train: drift times, then the ion-parallel form's partials (one block
+    // row per bunch for every block holding an ion made ahead of it), or the
+    // relay's cloud when those would not fit
+    double *dt = (double *)calloc(G, sizeof(double));
+    for (int ib = 1; ib < G; ib++) dt[ib] = fabs((double)(w->bucket_h[ib] - w->bucket_h[ib - 1])) * (c->T0 / c->h);
+    WA(w->d_dtime, G);
+    WUP(w->d_dtime, dt, G);
+    CK(hipStreamSynchronize(w->stream));
+    free(dt);
+    const int NQ = 2 * S + 3;
+    const long long nI = (long long)(G - 1) * (long long)npts;
+    w->D.nblk_ions = (int)((nI + WS_WG - 1) / WS_WG);
+    long long *toff = (long long *)malloc(sizeof(long long) * G), ntp = 0;
+    for (int ib = 0; ib < G; ib++) {
+      toff[ib] = ntp;
+      long long nb = ((long long)ib * (long long)npts + WS_WG - 1) / WS_WG;
+      if (nb > w->D.nblk_ions) nb = w->D.nblk_ions;
+      ntp += nb * NQ;
+    }
+    w->serial = (flags & MBT_WS_SERIAL_TRAIN) || G < 2 || ntp * (long long)sizeof(double) > WS_TRAIN_PARTIAL_MAX;
+    if (!w->serial && P.fbii) {
+      WA(w->D.tpart, ntp);
+      WA(w->d_toff, G);
+      WUP(w->d_toff, toff, G);
+      CK(hipStreamSynchronize(w->stream));
+    } else if (P.fbii) {
+      w->D.cap = (long long)G * npts;
+      WA(w->D.cl, (size_t)5 * w->D.cap);
+      w->maxblk = (int)((w->D.cap + WS_WG - 1) / WS_WG);
+      WA(w->D.part, (size_t)w->maxblk * S * 2);
+      WA(w->D.psum, (size_t)w->maxblk * 3);
+    }
+    free(toff);
     WA(w->D.done, 1);
     WA(w->D.cm, (size_t)WS_T * G * 4);
     WA(w->D.mon, (size_t)WS_T * G * 18);
@@ -752,6 +923,8 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
   w->D.Ibunch = w->d_Ibunch;
   w->D.nelec = w->d_nelec;
   w->D.convion = w->d_convion;
+  w->D.dtime = w->d_dtime;
+  w->D.toff = w->d_toff;
   *out = w;
   return MBT_OK;
 }
@@ -786,7 +959,12 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
       CK(hipGetLastError());
       hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
       CK(hipGetLastError());
-      if (w->P.fbii) {
+      if (w->P.fbii && !w->serial) {
+        hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), 0, w->stream, w->P, w->D);
+        CK(hipGetLastError());
+        hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
+        CK(hipGetLastError());
+      } else if (w->P.fbii) {
         hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
         CK(hipGetLastError());
         for (int ib = 1; ib < G; ib++) {

@@ -5,7 +5,9 @@ functions) on the same inputs.
 Sources of difference, all at the level of rounding: the device's sin in the
 synchrotron step and exp/sin/cos inside the Faddeeva function (within a few
 ulp of glibc's), and the summation order of the ion-field sums over the cloud
-(a block tree instead of a sequential loop).  Every output is compared as
+(a block tree instead of a sequential loop).  Both forms of the ion train are
+checked: one lane per ion over every later bunch (the default) and the
+reference's bunch-by-bunch relay (MBT_WS_SERIAL_TRAIN).  Every output is compared as
 max |gpu - oracle| / max |oracle| over the run, per quantity, against RTOL.
 Runs are bitwise reproducible (fixed-order reductions).
 """
@@ -25,16 +27,17 @@ def scaled_err(g, r):
     return float(np.max(np.abs(g - r)) / max(np.max(np.abs(r)), 1e-300))
 
 
+@pytest.mark.parametrize("serial", [False, True], ids=["ion_parallel", "relay"])
 @pytest.mark.parametrize("name", sorted(WS_VARIANTS))
-def test_ws_matches_oracle(tmp_path, name):
+def test_ws_matches_oracle(tmp_path, name, serial):
     path, mode = write_ws_variant(name, str(tmp_path))
-    print("case", name, flush=True)
+    print("case", name, "serial train" if serial else "ion-parallel train", flush=True)
     cfg = m.Config(path, str(tmp_path) + "/")
     n = cfg.geti("NrevTot")
     orc = o.OracleWS(cfg, mode)
     ocm, omon, oion, ozcm, rc = orc.track(n)
     assert rc == 0
-    g = m.WeakStrong(cfg, bunch_currents=bool(mode))
+    g = m.WeakStrong(cfg, bunch_currents=bool(mode), serial_train=serial)
     gcm, gmon, gion, gzcm = g.track(n)
     errs = {"cm": scaled_err(gcm, ocm), "zcm": scaled_err(gzcm, ozcm)}
     mon_t = [t for t in range(n) if t % cfg.geti("NrevMon") == 0]
