@@ -126,3 +126,33 @@ def test_chunk_sizes_match_oracle(tmp_path, items):
     hc_o, tr_o = orc.histograms()
     hc_g, tr_g = g.histograms()
     assert np.array_equal(hc_o, hc_g) and np.array_equal(tr_o, tr_g)
+
+
+def test_qe_statistics_match_reference_stream(tmp_path):
+    """Quantum excitation over 1,200 turns of relaxation (tests/test_qe_statistics.py
+    explains the configuration and tolerances): the GPU run follows the oracle's
+    counter-based stream turn by turn, and its sigma^2 curves meet the statistical
+    bounds against the reference's own glibc stream."""
+    from test_qe_statistics import CURVE_TOL, EQ_TOL
+    from variants import QE_STAT_TURNS, qe_stat_compare, sigma2_curves, write_qe_stat
+    d = str(tmp_path)
+    cfg = m.Config(write_qe_stat(d), d + "/")
+    print("case qe statistics", flush=True)
+    g = m.Ring(cfg)
+    g.prepare()
+    gst, _, _ = g.track(QE_STAT_TURNS)
+    g.close()
+    sg = sigma2_curves(gst)
+    curves = {}
+    for name, rng in (("philox", o.RNG_PHILOX), ("reference", o.RNG_REFERENCE)):
+        orc = o.OracleRing(cfg, rng)
+        orc.prepare()
+        st, _, _ = orc.track(QE_STAT_TURNS)
+        orc.close()
+        curves[name] = sigma2_curves(st)
+    same = np.abs(sg - curves["philox"]) / curves["philox"]
+    eq, curve = qe_stat_compare(curves["reference"], sg)
+    print("gpu vs oracle (same stream) max rel %.3g; vs reference stream: equilibrium %s per-turn %s" % (
+        same.max(), np.round(eq, 4), np.round(curve, 4)), flush=True)
+    assert same.max() <= 1e-9
+    assert np.all(eq <= EQ_TOL) and np.all(curve <= CURVE_TOL)

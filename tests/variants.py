@@ -122,3 +122,39 @@ def write_ws_variant(name, d):
     cg.filling_file(fill, WS_FILL[h], out=())
     text = cg.ring_conf(**v).replace("@FILL@", os.path.abspath(fill))
     return cg.write(os.path.join(d, name + ".conf"), text), mode
+
+
+# ---- quantum-excitation statistics: 8 bunches in h = 8, every initial sigma
+# twice its nominal value, strong damping (taue = 100 turns, U0 = 4 MeV: about
+# 750 turns of transverse amplitude damping through the RF focusing term), so
+# every plane relaxes towards its equilibrium within the run
+QE_STAT_TURNS = 1200
+
+
+def write_qe_stat(d, np_=4000, turns=QE_STAT_TURNS, seed=0, **ring):
+    os.makedirs(d, exist_ok=True)
+    T0 = 354.0 / 2.997925e8
+    r = {"Lc": cg.small_ring(8), "taue": 100 * T0 * 1e3, "Iring": 1.0, "U0": 4000.0}
+    r.update(ring)
+    text = cg.ring_conf(sections={
+        "tracking": {"NrevTot": turns, "NrevMon": 10, "EnableQuantum": 1},
+        "ring": r,
+        "distribution": {"Np": np_, "sgm_xtau": 0.02, "sgm_xeps": 2.0e-3, "sgm_xx": 0.2, "sgm_xp": 0.02,
+                         "sgm_zz": 0.02, "sgm_zp": 0.002, "xxCM_offset": 0.0, "zzCM_offset": 0.0,
+                         "iseed_LON": 11 + seed, "iseed_HOR": 22 + seed, "iseed_VER": 33 + seed}})
+    return cg.write(os.path.join(d, "qe_stat.conf"), text)
+
+
+def sigma2_curves(stats):
+    """Bunch-averaged rms^2 per turn and plane, [turns][3], from stats [turns][G][12]."""
+    import numpy as np
+    return (np.asarray(stats)[..., 3:6] ** 2).mean(axis=1)
+
+
+def qe_stat_compare(a, b):
+    """(relative difference of the last-third means, max per-turn relative
+    difference), each per plane, of two sigma2_curves outputs."""
+    import numpy as np
+    w = slice(-(len(a) // 3), None)
+    eq = np.abs(a[w].mean(0) - b[w].mean(0)) / a[w].mean(0)
+    return eq, (np.abs(a - b) / a).max(0)
