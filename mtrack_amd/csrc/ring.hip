@@ -84,6 +84,7 @@
 // reference's formulas so that every constant equals the reference's.
 struct DevParams {
   int ring, Nb, h, Nc, Nmlt;
+  unsigned long long zero_bits; // always 0: the timing pass's stores xor with it
   int bunch_off;  // first global bunch of the ring
   int bucket_off; // first global bucket of the ring
   long long rw_off, st_off; // offsets into the RW history and sqrt table
@@ -480,6 +481,7 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
     // coordinates stream through once per turn: nontemporal loads and stores
     double tau = LDC(pt + i), del = LDC(pd + i), x = LDC(px + i), xp = LDC(pxp + i), z = LDC(pz + i),
            zp = LDC(pzp + i);
+    const double in[6] = {tau, del, x, xp, z, zp}; // kept only by the timing pass
     if (rw_h) xp += rwx;
     if (rw_v) zp += rwz;
     if (MODE == MODE_TURN || MODE == MODE_TUNE) {
@@ -551,12 +553,15 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
       }
     }
     if (MODE == MODE_TUNE) {
-      STC(LDC(pt + i), pt + i);
-      STC(LDC(pd + i), pd + i);
-      STC(LDC(px + i), px + i);
-      STC(LDC(pxp + i), pxp + i);
-      STC(LDC(pz + i), pz + i);
-      STC(LDC(pzp + i), pzp + i);
+      // a value the compiler cannot prove equal to the loaded one, so the
+      // stores (and their traffic) stay; the bits written are the bits read
+      const unsigned long long zb = P.zero_bits;
+      STC(__longlong_as_double(__double_as_longlong(in[0]) ^ zb), pt + i);
+      STC(__longlong_as_double(__double_as_longlong(in[1]) ^ zb), pd + i);
+      STC(__longlong_as_double(__double_as_longlong(in[2]) ^ zb), px + i);
+      STC(__longlong_as_double(__double_as_longlong(in[3]) ^ zb), pxp + i);
+      STC(__longlong_as_double(__double_as_longlong(in[4]) ^ zb), pz + i);
+      STC(__longlong_as_double(__double_as_longlong(in[5]) ^ zb), pzp + i);
     } else if (MODE != MODE_STATS) {
       STC(tau, pt + i);
       STC(del, pd + i);

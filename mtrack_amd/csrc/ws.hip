@@ -112,6 +112,32 @@ __device__ __forceinline__ double wsum64(double v)
   return v;
 }
 
+// Eight sums over the 64 lanes of a wave at once: three halving exchanges (xor
+// 32, 16, 8) split the eight values between lane groups, three more (xor 4, 2,
+// 1) finish each; lanes 8k .. 8k+7 return the total of v[k].  Ten exchanges
+// instead of 48 for eight separate reductions; a fixed tree, so reproducible.
+__device__ __forceinline__ double wsum64x8(const double v[8])
+{
+  const int lane = threadIdx.x & 63;
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+  double w[4], x[2];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double mine = b5 ? v[4 + k] : v[k], other = b5 ? v[k] : v[4 + k];
+    w[k] = mine + __shfl_xor(other, 32, 64);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const double mine = b4 ? w[2 + k] : w[k], other = b4 ? w[k] : w[2 + k];
+    x[k] = mine + __shfl_xor(other, 16, 64);
+  }
+  double y = (b3 ? x[1] : x[0]) + __shfl_xor(b3 ? x[0] : x[1], 8, 64);
+  y += __shfl_xor(y, 4, 64);
+  y += __shfl_xor(y, 2, 64);
+  y += __shfl_xor(y, 1, 64);
+  return y;
+}
+
 // bunch_strong_updateCM, src/bunch_strong.c:221-266 (sequential, one lane)
 __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, double &pos, double &slope)
 {
@@ -556,26 +582,31 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
       x = x + dt * xd;
       y = y + dt * yd;
     }
-    for (int c = 0; c < S; c++) {
-      double sx = 0.0, sy = 0.0;
-      if (act) {
-        double Ex, Ey;
-        ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
-        const double ax = P.factF * Ex;
-        xd = xd + P.deltT * ax;
-        x = x + P.deltT * xd;
-        const double ay = P.factF * Ey;
-        yd = yd + P.deltT * ay;
-        y = y + P.deltT * yd;
-        sx = Nri * Ex;
-        sy = Nri * Ey;
+    // four slices at a time: their eight field sums reduced together
+    for (int c0 = 0; c0 < S; c0 += 4) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int c = c0 + k;
+        double sx = 0.0, sy = 0.0;
+        if (act && c < S) {
+          double Ex, Ey;
+          ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+          const double ax = P.factF * Ex;
+          xd = xd + P.deltT * ax;
+          x = x + P.deltT * xd;
+          const double ay = P.factF * Ey;
+          yd = yd + P.deltT * ay;
+          y = y + P.deltT * yd;
+          sx = Nri * Ex;
+          sy = Nri * Ey;
+        }
+        v[2 * k] = sx;
+        v[2 * k + 1] = sy;
       }
-      sx = wsum64(sx);
-      sy = wsum64(sy);
-      if (lane == 0) {
-        wred[2 * c][wv] = sx;
-        wred[2 * c + 1][wv] = sy;
-      }
+      const double t = wsum64x8(v);
+      const int idx = lane >> 3, c = c0 + (idx >> 1);
+      if ((lane & 7) == 0 && c < S) wred[2 * c + (idx & 1)][wv] = t;
     }
     double s0 = act ? Nri * x : 0.0, s1 = act ? Nri * y : 0.0, s2 = act ? Nri : 0.0;
     s0 = wsum64(s0);
