@@ -21,7 +21,12 @@ a bounded number of turns.
 bunch count, macroparticle count, current and chromaticity; RF beam-loading
 feedback on) tracked together by the same kernels, FP64, with its own
 throughput and k_turn roofline; the work decomposition is chosen by the
-autotuner (mbt_ring_autotune) first.  `fbii` is the weak-strong fast beam-ion
+autotuner (mbt_ring_autotune) first; `batched.mixed` repeats it in mixed precision
+(FP32 wake convolutions, FP32 hi/lo MFMA resistive-wall GEMM) with its
+statistics error against the FP64 run and its ratio to the plain FP64 small
+ring.  `c1` is that small ring (64 bunches x 1,000 macroparticles, QE on) in
+FP64 and mixed precision, with the mixed run's errors and the growth rates of
+a resistive-wall instability in both.  `fbii` is the weak-strong fast beam-ion
 workload (356-bunch train in h = 416), in ion-slice interactions/s, with the
 reference's own functions timed on a 48-bunch train beside it.
 """
@@ -39,6 +44,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_MP_TURN = 96.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_turn_production_traffic.txt")
+TRAFFIC_FILE_BATCHED = os.path.join(ROOT, "profiles", "k_turn_batched_traffic.txt")
 
 
 def production_conf(d, np_, turns):
@@ -48,13 +54,13 @@ def production_conf(d, np_, turns):
     return path
 
 
-def read_traffic():
+def read_traffic(path=TRAFFIC_FILE):
     """HBM bytes per k_turn launch measured with rocprofv3 --pmc (scaled by the
     copy-kernel calibration), from the committed profile summary."""
-    if not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(path):
         return None
     vals = {}
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             p = line.split()
             if len(p) == 2 and not line.startswith("#"):
@@ -90,16 +96,86 @@ def cpu_baseline(np_, turns=2, timeout=240):
                 "sample": sample}
 
 
-def batched(steps, warmup, nrings, seed, chunk=0):
+def stat_errors(g, r):
+    """Largest error of per-bunch statistics [..., 12]: centroids relative to
+    the rms of the same coordinate, rms relative to itself."""
+    import numpy as np
+    g, r = np.asarray(g), np.asarray(r)
+    e = 0.0
+    for p in range(3):
+        for mc, sc in ((p, 3 + p), (6 + p, 9 + p)):
+            scale = np.maximum(np.abs(r[..., sc]), 1e-300)
+            e = max(e, float(np.max(np.abs(g[..., mc] - r[..., mc]) / scale)),
+                    float(np.max(np.abs(g[..., sc] - r[..., sc]) / scale)))
+    return e
+
+
+def growth_rates(am, nrev_mon, first):
+    """Slope per turn of log(bunch-averaged centroid action), HOR and VER, over
+    the monitor turns from `first` on."""
+    import numpy as np
+    turns = np.arange(len(am))
+    mon = (turns + 1) % nrev_mon == 0
+    a = am[mon][..., 3:6].mean(1)
+    t = turns[mon]
+    sel = t >= first
+    return [float(np.polyfit(t[sel], np.log(a[sel, p]), 1)[0]) for p in (1, 2)]
+
+
+def c1(turns=2000):
+    """The small multi-bunch ring (64 bunches x 1,000 macroparticles, QE on,
+    copper RW, broadband resonator; confgen.wl_c1): the plain FP64 GPU path and
+    the mixed-precision one, each timed over `turns` turns from the same start,
+    with the mixed run's errors against FP64; growth rates of the same ring
+    driven unstable (400 mA, 6 mm RW radius, 1,000 turns) in both precisions."""
+    import mtrack_amd as m
+    from mtrack_amd import confgen
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        cfg = m.Config(confgen.wl_c1(d, turns=turns), d + "/")
+        n = 64 * cfg.geti("Np")
+        st = {}
+        for mixed in (False, True):
+            g = m.Ring(cfg, mixed=mixed)
+            g.prepare()
+            t0 = time.perf_counter()
+            st[mixed], _, _ = g.track(turns)
+            sec = time.perf_counter() - t0
+            g.close()
+            out["mixed" if mixed else "fp64"] = {"value": n * turns / sec, "unit": "macroparticle-turns/s",
+                                                 "ms_per_step": 1e3 * sec / turns}
+        out["mixed"]["stat_error_vs_fp64"] = stat_errors(st[True], st[False])
+        cfg = m.Config(confgen.wl_c1(d, turns=1000, iring=400.0, beff=0.006), d + "/")
+        rates = {}
+        for mixed in (False, True):
+            g = m.Ring(cfg, mixed=mixed)
+            g.prepare()
+            _, am, _ = g.track(1000)
+            g.close()
+            rates[mixed] = growth_rates(am, 10, 500)
+        out["growth_rate_per_turn"] = {"fp64": rates[False], "mixed": rates[True],
+                                       "rel_error": [abs(a - b) / abs(b) for a, b in zip(rates[True], rates[False])],
+                                       "config": "400 mA, 6 mm RW radius, 1,000 turns, fit over turns 500-999"}
+    out["steps"] = turns
+    out["config"] = {"workload": "64 bunches (buckets int(6.5 k) of h 416) x 1,000 macroparticles, QE on, "
+                                 "copper RW 12 mm over 19 turns, 10 GHz broadband resonator in 50 slices, 208 mA",
+                     "precision": "mixed: FP32 wake convolutions, FP32 hi/lo MFMA RW GEMM with FP64 chunk sums; "
+                                  "coordinates, phase advance and RF recurrence FP64"}
+    return out
+
+
+def batched(steps, warmup, nrings, seed, chunk=0, mixed=False):
     """The batched ensemble: nrings independent rings (bunch and macroparticle
     counts log-uniform, currents and chromaticities uniform, RF beam-loading
-    feedback on; mtrack_amd.confgen.wl_ensemble) tracked together, FP64."""
+    feedback on; mtrack_amd.confgen.wl_ensemble) tracked together, FP64 (or
+    mixed precision).  Returns the bench object and the statistics of the last
+    timed turns."""
     import mtrack_amd as m
     from mtrack_amd import confgen
     with tempfile.TemporaryDirectory() as d:
         paths = confgen.wl_ensemble(d, nrings=nrings, seed=seed)
         cfgs = [m.Config(p, d + "/") for p in paths]
-        ens = m.Ensemble(cfgs, init_dist=False)
+        ens = m.Ensemble(cfgs, init_dist=False, mixed=mixed)
         # every bunch of a ring starts from the ring's first bunch as the
         # reference generator makes it (the same Gaussian for all its bunches)
         for k, c in enumerate(cfgs):
@@ -120,7 +196,7 @@ def batched(steps, warmup, nrings, seed, chunk=0):
         g1 = time.perf_counter()
         ens.set_timing(True)
         t0 = time.perf_counter()
-        ens.track(steps)
+        last_stats, _, _ = ens.track(steps)
         t1 = time.perf_counter()
         ms, launches = ens.kernel_times()
         bytes_turn = ens.bytes_per_turn()
@@ -128,7 +204,8 @@ def batched(steps, warmup, nrings, seed, chunk=0):
     sec = t1 - t0
     k_turn_ms = ms[0] / max(launches[0], 1)
     achieved = bytes_turn / (k_turn_ms * 1e-3) / 1e9
-    return {"value": N * steps / sec, "unit": "macroparticle-turns/s", "dtype": "f64", "steps": steps,
+    return {"value": N * steps / sec, "unit": "macroparticle-turns/s", "dtype": "mixed" if mixed else "f64",
+            "steps": steps,
             "warmup": warmup, "ms_per_step": 1e3 * sec / steps,
             "config": {"workload": "batched ensemble of %d independent rings in one launch per kernel: bunches "
                                    "32-416 and macroparticles 1e4-1e5 log-uniform, 0.1-5 mA/bunch, chromaticity "
@@ -139,9 +216,9 @@ def batched(steps, warmup, nrings, seed, chunk=0):
             "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                    "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(TRAFFIC_FILE_BATCHED)},
             "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
-                               "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)}}
+                               "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)}}, last_stats
 
 
 def fbii_conf(d, train=356, np_=10000, turns=2000):
@@ -230,6 +307,9 @@ def main():
     ap.add_argument("--batched-seed", type=int, default=20240607)
     ap.add_argument("--batched-chunk", type=int, default=0,
                     help="work items per lane for the batched line (0: autotune)")
+    ap.add_argument("--batched-mixed", action="store_true", help="with --no-production: the batched line in mixed precision")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision batched run")
+    ap.add_argument("--no-c1", action="store_true", help="skip the small-ring FP64 / mixed line")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production line (profiling of the other lines only)")
     ap.add_argument("--no-fbii", action="store_true", help="skip the fast beam-ion line")
@@ -245,7 +325,8 @@ def main():
     if a.no_production:
         out = {}
         if not a.no_batched:
-            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
+            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk,
+                                     a.batched_mixed)[0]
         if not a.no_fbii:
             out["fbii"] = fbii(a.fbii_steps, 2, cpu=False)
         print(json.dumps(out))
@@ -299,9 +380,23 @@ def main():
                            "graphs": not a.no_graphs, "ms_per_step": 1e3 * (g1 - g0) / a.steps,
                            "value": nb * npb * a.steps / (g1 - g0)},
     }
+    if not a.no_c1:
+        try:
+            out["c1"] = c1()
+        except Exception as e:
+            out["c1"] = {"value": None, "error": str(e)[:300]}
     if not a.no_batched:
         try:
-            out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
+            out["batched"], st64 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
+            if not a.no_mixed:
+                mix, st32 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed,
+                                    out["batched"]["config"]["chunk_items"], mixed=True)
+                mix["stat_error_vs_fp64"] = stat_errors(st32, st64)
+                if isinstance(out.get("c1"), dict) and out["c1"].get("fp64"):
+                    mix["vs_plain_fp64_c1"] = mix["value"] / out["c1"]["fp64"]["value"]
+                for k in ("config", "roofline"):
+                    mix.pop(k, None)
+                out["batched"]["mixed"] = mix
         except Exception as e:  # reported beside the FP64 line, never replaces it
             out["batched"] = {"value": None, "error": str(e)[:300]}
     if not a.no_fbii:

@@ -194,6 +194,11 @@ typedef struct mbt_ring mbt_ring;
                                    HIP graphs of whole blocks of turns */
 #define MBT_RING_NO_RW_GEMM 4   /* ensembles: keep the per-bunch resistive-wall sum even where the
                                    rings share one tap table (otherwise one FP64 MFMA GEMM) */
+#define MBT_RING_MIXED 8        /* mixed precision: wake (selffield) convolutions in FP32 and the
+                                   resistive-wall sum as an FP32 MFMA GEMM with hi/lo operand
+                                   splitting and FP64 sums of FP32 chunks (any number of rings
+                                   sharing a tap table); coordinates, phase advance and the
+                                   LR / RF-feedback recurrence stay FP64 */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);
@@ -281,6 +286,9 @@ typedef struct mbt_ws mbt_ws;
  * the relay is also taken when the default's per-bunch partials would exceed
  * 8 GiB).  Both give the same sums up to their summation order. */
 #define MBT_WS_SERIAL_TRAIN 2
+/* flags: mixed precision -- the Faddeeva function of the Bassetti-Erskine field
+ * tables in FP32; centroids, ions and the field arithmetic around it stay FP64 */
+#define MBT_WS_MIXED 4
 int mbt_ws_create(const mbt_config *cfg, int device, int flags, mbt_ws **out);
 void mbt_ws_destroy(mbt_ws *w);
 /* nturns turns; per turn (any output may be NULL):

@@ -156,3 +156,26 @@ def wl_ensemble(d, nrings=256, seed=20240607, turns=10000, np_range=(10000, 1000
                             ("rf_feedback", {"resonator": 1, "averaging_length": 416})])
         paths.append(write(os.path.join(d, "ring_%03d.conf" % k), text))
     return paths
+
+
+def wl_c1(d, turns=2000, np_=1000, iring=208.0, beff=0.012, ncell=50):
+    """The small multi-bunch ring: 64 bunches in buckets int(6.5 k) of h = 416,
+    np_ macroparticles each, quantum excitation on (damping 10k / 10k / 5k
+    turns), chromaticity 1 / 1, copper resistive wall (rho 1.7e-8 Ohm m, radius
+    beff) over Nmlt = 19 past turns, a 10 GHz broadband resonator in Ncell
+    slices, Iring [mA] shared equally.  Writes the filling file and the .conf
+    into d and returns the .conf path."""
+    os.makedirs(d, exist_ok=True)
+    fill = os.path.join(d, "fill_c1.txt")
+    filling_file(fill, [int(k * 6.5) for k in range(64)], out=(0,))
+    text = ring_conf(
+        sections={
+            "selffield": {"Ncell": ncell, "Nsigma": 5.0},
+            "tracking": {"NrevTot": turns, "NrevMon": 10, "EnableRW_long": 1, "Nmlt": 19, "EnableQuantum": 1,
+                         "filling": fill},
+            "ring": {"Iring": iring, "rhorw": 1.7e-8, "beffL1": beff, "beffL2": beff, "beffH3": beff,
+                     "beffV3": beff},
+            "distribution": {"Np": np_},
+        },
+        extra_sections=[("resonator_1", {"Rs": 5000.0, "fres": 10.0, "Qfactor": 1.0, "plane": "LON"})])
+    return write(os.path.join(d, "c1.conf"), text)

@@ -85,6 +85,7 @@
 struct DevParams {
   int ring, Nb, h, Nc, Nmlt;
   unsigned long long zero_bits; // always 0: the timing pass's stores xor with it
+  int mixed;                    // FP32 wake convolutions (MBT_RING_MIXED)
   int bunch_off;  // first global bunch of the ring
   int bucket_off; // first global bucket of the ring
   long long rw_off, st_off; // offsets into the RW history and sqrt table
@@ -866,7 +867,13 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     double GL1 = 0.0, GV = 0.0, GH = 0.0;
     if (P.convL && i >= icmin) {
       double Gn = 0.0;
-      for (int k = i; k >= icmin; k--) Gn += fnp[k] * Gl[i - k];
+      if (P.mixed) {
+        float g = 0.0f;
+        for (int k = i; k >= icmin; k--) g += (float)fnp[k] * (float)Gl[i - k];
+        Gn = (double)g;
+      } else {
+        for (int k = i; k >= icmin; k--) Gn += fnp[k] * Gl[i - k];
+      }
       GL1 = 1 * Gn;
       // resistive and inductive smoothing, src/transform_weak.c:670-683; reads
       // outside [0, Ncell) count as zero
@@ -884,12 +891,24 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     }
     if (P.kickV) {
       double Gn = 0.0;
-      for (int k = i; k >= 0; k--) Gn += dV[k] * Gv[i - k];
+      if (P.mixed) {
+        float g = 0.0f;
+        for (int k = i; k >= 0; k--) g += (float)dV[k] * (float)Gv[i - k];
+        Gn = (double)g;
+      } else {
+        for (int k = i; k >= 0; k--) Gn += dV[k] * Gv[i - k];
+      }
       GV = 1 * Gn;
     }
     if (P.kickH && i >= 1) {
       double Gn = 0.0;
-      for (int k = i; k >= 0; k--) Gn += dH[k] * Gh[i - k];
+      if (P.mixed) {
+        float g = 0.0f;
+        for (int k = i; k >= 0; k--) g += (float)dH[k] * (float)Gh[i - k];
+        Gn = (double)g;
+      } else {
+        for (int k = i; k >= 0; k--) Gn += dH[k] * Gh[i - k];
+      }
       GH = 1 * Gn;
     }
     R.sfL[(size_t)b * Nc + i] = factG * GL1;
@@ -1076,6 +1095,28 @@ __global__ __launch_bounds__(256) void k_rwg_y(RWGemm Gm, const DevParams *__res
   Gm.Y[e] = v;
 }
 
+// Sums the four waves' tiles and writes the pending kicks of the tile's bunches
+// (the ring's fNp and RW constant applied).
+__device__ __forceinline__ void rwg_store(const RWGemm &Gm, const DevParams *__restrict__ Pr,
+                                          const TurnState *__restrict__ tsr, double *pending,
+                                          double (*acc_s)[16][17], int ti, int tj)
+{
+  __syncthreads();
+  const int r = threadIdx.x >> 4, cc = threadIdx.x & 15;
+  const double sum = ((acc_s[0][r][cc] + acc_s[1][r][cc]) + acc_s[2][r][cc]) + acc_s[3][r][cc];
+  const int kb = ti * 16 + r, col = tj * 16 + cc;
+  if (kb < Gm.h && col < Gm.N) {
+    const int ring = Gm.col_ring[col >> 1], pl = col & 1;
+    const DevParams &P = Pr[ring];
+    const int g = Gm.bucket_bunch[P.bucket_off + kb];
+    if (g >= 0) {
+      const int trig = tsr[ring].m < 50 ? 1 : 0;
+      const bool on = pl == 0 ? P.track_h : P.track_v;
+      pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
+    }
+  }
+}
+
 // one block (4 waves) per 16 x 16 output tile; wave w covers a quarter of K
 __global__ __launch_bounds__(256) void k_rwg_mfma(RWGemm Gm, const DevParams *__restrict__ Pr,
                                                 const TurnState *__restrict__ tsr, double *pending)
@@ -1099,22 +1140,48 @@ __global__ __launch_bounds__(256) void k_rwg_mfma(RWGemm Gm, const DevParams *__
   // f64 16x16x4 result layout: element v of lane l is D[row = l / 16 + 4 v][col = l % 16]
 #pragma unroll
   for (int v = 0; v < 4; v++) acc_s[wv][(lane >> 4) + 4 * v][lane & 15] = acc[v];
-  __syncthreads();
-  if (threadIdx.x < 256) {
-    const int r = threadIdx.x >> 4, cc = threadIdx.x & 15;
-    const double sum = ((acc_s[0][r][cc] + acc_s[1][r][cc]) + acc_s[2][r][cc]) + acc_s[3][r][cc];
-    const int kb = ti * 16 + r, col = tj * 16 + cc;
-    if (kb < Gm.h && col < Gm.N) {
-      const int ring = Gm.col_ring[col >> 1], pl = col & 1;
-      const DevParams &P = Pr[ring];
-      const int g = Gm.bucket_bunch[P.bucket_off + kb];
-      if (g >= 0) {
-        const int trig = tsr[ring].m < 50 ? 1 : 0;
-        const bool on = pl == 0 ? P.track_h : P.track_v;
-        pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
-      }
+  rwg_store(Gm, Pr, tsr, pending, acc_s, ti, tj);
+}
+
+// The same GEMM in FP32 MFMA (MBT_RING_MIXED): each operand split into a high
+// and a low float, three products hi*hi + hi*lo + lo*hi per step, summed in
+// FP32 over RWG32_CHUNK steps and then into an FP64 accumulator.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+#define RWG32_CHUNK 64
+
+__global__ __launch_bounds__(256) void k_rwg_mfma32(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                                  const TurnState *__restrict__ tsr, double *pending)
+{
+  __shared__ double acc_s[4][16][17];
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = ti * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  const int j = tj * 16 + (lane & 15);
+  const int ksteps = Gm.K / 4, per = (ksteps + 3) / 4;
+  const int s0 = wv * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
+  double accd[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c0 = s0; c0 < s1; c0 += RWG32_CHUNK) {
+    const int c1 = c0 + RWG32_CHUNK < s1 ? c0 + RWG32_CHUNK : s1;
+    mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int st = c0; st < c1; st++) {
+      const int a = st * 4 + kq;
+      const int m = a - i;
+      const double av = (i < Gm.h && m >= 1 && m <= Gm.M) ? Gm.c[m] : 0.0;
+      const double bv = j < Gm.N ? Gm.Y[(size_t)a * Gm.N + j] : 0.0;
+      const float ah = (float)av, al = (float)(av - (double)ah);
+      const float bh = (float)bv, bl = (float)(bv - (double)bh);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc, 0, 0, 0);
     }
+#pragma unroll
+    for (int v = 0; v < 4; v++) accd[v] += (double)acc[v];
   }
+  // f32 16x16x4 result layout: element v of lane l is D[row = 4 (l / 16) + v][col = l % 16]
+#pragma unroll
+  for (int v = 0; v < 4; v++) acc_s[wv][4 * (lane >> 4) + v][lane & 15] = accd[v];
+  rwg_store(Gm, Pr, tsr, pending, acc_s, ti, tj);
 }
 
 // Turn end, one block per ring, for turns without a resistive-wall pass;
@@ -1398,6 +1465,7 @@ struct mbt_ring {
   long long Ntot;   // macroparticles
   long long Nalloc; // coordinate slots: bunches padded to BUNCH_ALIGN
   int rwg_on;        // RW sum as one GEMM over the ensemble's RW rings
+  int mixed;         // MBT_RING_MIXED: FP32 wake convolutions, FP32 hi/lo RW GEMM
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
   double *rwg_c, *rwg_Y;
@@ -1728,7 +1796,7 @@ static int setup_rw_gemm(mbt_ring *r)
       return MBT_OK; // rings with different taps: the per-bunch sum
     nrw++;
   }
-  if (nrw < RWG_MIN_RINGS) return MBT_OK;
+  if (nrw < (r->mixed ? 1 : RWG_MIN_RINGS)) return MBT_OK;
   RWGemm &g = r->rwg;
   g.h = h;
   g.M = (Nmlt + 1) * h;
@@ -1791,6 +1859,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   r->device = device;
   r->rh = (RingHost *)calloc(R, sizeof(RingHost));
   r->Ph = (DevParams *)calloc(R, sizeof(DevParams));
+  r->mixed = (flags & MBT_RING_MIXED) ? 1 : 0;
   for (int k = 0; k < R; k++) copy_cfg(&r->rh[k], cfgs[k]);
   CK(hipSetDevice(device));
   // a blocking stream: ordered after work on the null stream (allocation-time
@@ -1815,6 +1884,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     }
     DevParams &P = r->Ph[k];
     fill_params(c, hh->planes, P);
+    P.mixed = (flags & MBT_RING_MIXED) ? 1 : 0;
     P.ring = k;
     P.bunch_off = G;
     P.bucket_off = H;
@@ -2320,8 +2390,12 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
     const long long tot = (long long)r->rwg.K * r->rwg.N;
     hipLaunchKernelGGL(k_rwg_y, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
                        r->rwhist, r->Ibunch, r->nfFill);
-    hipLaunchKernelGGL(k_rwg_mfma, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream, r->rwg,
-                       r->P, r->ts, r->pending);
+    if (r->mixed)
+      hipLaunchKernelGGL(k_rwg_mfma32, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream,
+                         r->rwg, r->P, r->ts, r->pending);
+    else
+      hipLaunchKernelGGL(k_rwg_mfma, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream,
+                         r->rwg, r->P, r->ts, r->pending);
     CK(hipGetLastError());
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                        !lr_launch);

@@ -72,6 +72,7 @@ extern "C" void mbt_ws_grid(const mbt_config *c, double *agLOCx, double *agLOCz)
 
 struct WSP {
   int S, G, h, Nmlt, NmultiT, fbii, rw, track_h, track_v, track[3], Nions;
+  int mixed; // Faddeeva function of the field tables in FP32 (MBT_WS_MIXED)
   double eps, wrf, phai0, Urad, T0, ac;                 // synchrotron step
   double gamma1[3], alpha1[3], beta1[3];                // statistics
   double v11, v12, v21, v22, h11, h12, h21, h22;        // one-turn rotation (glibc cos/sin)
@@ -283,8 +284,18 @@ __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev
     const double Dy = msz * P.aAz - zz[cell];
     const double yi1 = Dy / P.factBE, yi2 = yi1 * P.sxx / P.szz;
     double u1, v1, u2, v2;
-    mbt_wofz_dev<double>(xi1, yi1, u1, v1);
-    mbt_wofz_dev<double>(xi2, yi2, u2, v2);
+    if (P.mixed) {
+      float a1, b1, a2, b2;
+      mbt_wofz_dev<float>((float)xi1, (float)yi1, a1, b1);
+      mbt_wofz_dev<float>((float)xi2, (float)yi2, a2, b2);
+      u1 = a1;
+      v1 = b1;
+      u2 = a2;
+      v2 = b2;
+    } else {
+      mbt_wofz_dev<double>(xi1, yi1, u1, v1);
+      mbt_wofz_dev<double>(xi2, yi2, u2, v2);
+    }
     const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
     const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
     ex_s[q] = Ex;
@@ -810,6 +821,7 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
   P.track_v = c->TrackPlane[2] != 0;
   for (int p = 0; p < 3; p++) P.track[p] = c->TrackPlane[p] != 0;
   P.Nions = S * 25;
+  P.mixed = (flags & MBT_WS_MIXED) ? 1 : 0;
   P.eps = c->Vrf0 / (c->E0 * MBT_FKILO);
   P.wrf = c->wrf;
   P.phai0 = c->phai0;
