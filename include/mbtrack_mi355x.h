@@ -159,6 +159,9 @@ typedef struct mbt_config {
 const char *mbt_last_error(void);
 
 /* ---- host configuration (C) ------------------------------------------------ */
+/* Reads a .conf file (read_conf_file, src/input.c:278-850) or a legacy .inp
+ * file (read_input_file, src/input.c:198-276), chosen by the extension as the
+ * reference does, then derives every machine parameter. */
 int mbt_config_read(const char *conf_path, const char *work_path, mbt_config *cfg);
 void mbt_config_free(mbt_config *cfg);
 /* Heap-allocated configuration for bindings; mbt_config_release frees both. */

@@ -174,14 +174,9 @@ static int plane_of(const char *s, int *pl)
   return 1;
 }
 
-/* src/input.c:278-850 */
-static int read_conf(conf_text *T_, mbt_config *c)
+/* defaults of read_input for either input format, src/input.c:166-182 */
+static void input_defaults(mbt_config *c)
 {
-  conf_text T = *T_;
-  char sec[32], str[MBT_PATH];
-  double tmp;
-
-  /* defaults of read_input, src/input.c:166-182 */
   c->Nslc = 0;
   c->fYokoyaV = 1.0;
   c->fYokoyaH = 1.0;
@@ -195,6 +190,103 @@ static int read_conf(conf_text *T_, mbt_config *c)
   c->filling = 5; /* SINGLE3 */
   c->triggerRW = 0;
   c->bunch_model = 0;
+}
+
+/* One data line of the legacy format (c_strng, src/input.c:89-111): lines whose
+ * first character is 'C' or 'c' are comments; the next line is the data.  Reads
+ * at most len-1 characters of it (the reference writes past its 81-byte buffer
+ * on longer lines, and never returns at end of file). */
+static int inp_line(FILE *fp, char *str, size_t len)
+{
+  int c0 = getc(fp);
+  while (c0 == 'C' || c0 == 'c') {
+    int c1;
+    do c1 = getc(fp);
+    while (c1 != '\n' && c1 != EOF);
+    if (c1 == EOF) return 0;
+    c0 = getc(fp);
+  }
+  if (c0 == EOF) return 0;
+  size_t i = 0;
+  while (c0 != '\n' && c0 != EOF) {
+    if (i + 1 < len) str[i++] = (char)c0;
+    c0 = getc(fp);
+  }
+  str[i] = 0;
+  return 1;
+}
+
+/* The legacy fixed-order input (read_input_file, src/input.c:198-276): 21 data
+ * lines; every quantity it does not name keeps the defaults above or zero. */
+static int read_inp(FILE *fp, mbt_config *c)
+{
+  char s[MBT_PATH];
+  input_defaults(c);
+#define LINE()                                                                                   \
+  do {                                                                                           \
+    if (!inp_line(fp, s, sizeof s)) {                                                            \
+      mbt_set_error("legacy input ends early");                                                  \
+      return 0;                                                                                  \
+    }                                                                                            \
+  } while (0)
+  LINE();
+  snprintf(c->jobtitle, 80, "%s", s);
+  LINE();
+  sscanf(s, "%lf", &c->E0);
+  LINE();
+  sscanf(s, "%lf%lf", &c->Lc, &c->rho0);
+  LINE();
+  sscanf(s, "%lf%lf%lf", &c->beta1[MBT_HOR], &c->beta1[MBT_VER], &c->dispH1);
+  LINE();
+  sscanf(s, "%lf%lf%lf", &c->alpha1[MBT_HOR], &c->alpha1[MBT_VER], &c->disppH1);
+  LINE();
+  sscanf(s, "%lf%lf", &c->emittanceH, &c->couplbeta);
+  LINE();
+  sscanf(s, "%lf%lf", &c->ac, &c->Je);
+  LINE();
+  sscanf(s, "%lf%lf", &c->Vrf0, &c->frf);
+  LINE();
+  sscanf(s, "%lf", &c->Iring);
+  LINE();
+  sscanf(s, "%lf%lf", &c->ApertureH, &c->ApertureV);
+  LINE();
+  sscanf(s, "%lf%lf", &c->Gzix, &c->Gziz);
+  LINE();
+  sscanf(s, "%d%d%d", &c->TrackPlane[0], &c->TrackPlane[1], &c->TrackPlane[2]);
+  LINE();
+  sscanf(s, "%ld%d%ld", &c->NrevTot, &c->Nmlt, &c->NrevMon);
+  c->NmultiT = c->Nmlt + 2;
+  /* no current scan; the reference leaves NrevScan 0 on this path and then
+   * takes every turn number modulo it -- the .conf reader's default instead */
+  c->NrevScan = c->NrevTot + 1;
+  LINE();
+  sscanf(s, "%d", &c->Np);
+  c->fNp = (double)c->Np;
+  LINE();
+  sscanf(s, "%d%d%d%d%d%d", &c->nGen[0], &c->iseed[0], &c->nGen[1], &c->iseed[1], &c->nGen[2], &c->iseed[2]);
+  for (int p = 0; p < 3; p++) {
+    LINE();
+    char w[MBT_PATH];
+    if (sscanf(s, "%1023s", w) == 1) snprintf(c->gendst_datafile[p], MBT_PATH, "%s", w);
+  }
+  LINE();
+  sscanf(s, "%lf%lf%lf%lf", &c->xtauCM_offset, &c->xepsCM_offset, &c->sgm_xtau, &c->sgm_xeps);
+  LINE();
+  sscanf(s, "%lf%lf%lf%lf", &c->xxCM_offset, &c->xpCM_offset, &c->sgm_xx, &c->sgm_xp);
+  LINE();
+  sscanf(s, "%lf%lf%lf%lf", &c->zzCM_offset, &c->zpCM_offset, &c->sgm_zz, &c->sgm_zp);
+#undef LINE
+  return 1;
+}
+
+/* src/input.c:278-850 */
+static int read_conf(conf_text *T_, mbt_config *c)
+{
+  conf_text T = *T_;
+  char sec[32], str[MBT_PATH];
+  double tmp;
+
+  input_defaults(c);
 
   snprintf(sec, sizeof sec, "tracking");
   T.print_errors = 1;
@@ -831,19 +923,32 @@ int mbt_config_read(const char *conf_path, const char *work_path, mbt_config *c)
     mbt_set_error("input file has no extension");
     return MBT_EINVAL;
   }
-  if (strcmp(dot + 1, "conf") != 0) {
+  const int inp = strcmp(dot + 1, "inp") == 0;
+  if (strcmp(dot + 1, "conf") != 0 && !inp) {
     fprintf(stderr, "ERROR: Unknown input file extension.\n");
-    mbt_set_error("only .conf input is supported (got %s)", dot + 1);
-    return strcmp(dot + 1, "inp") == 0 ? MBT_EUNSUPPORTED : MBT_EINVAL;
+    mbt_set_error("unknown input file extension %s (.conf or .inp)", dot + 1);
+    return MBT_EINVAL;
   }
-  conf_text T;
-  if (conf_load(conf_path, &T) != 0) {
-    fprintf(stderr, "ERROR: Cannot open file %s for reading.\n", conf_path);
-    mbt_set_error("cannot open %s", conf_path);
-    return MBT_EIO;
+  int ok;
+  if (inp) {
+    FILE *fp = fopen(conf_path, "r");
+    if (!fp) {
+      fprintf(stderr, "ERROR: Cannot open file %s for reading.\n", conf_path);
+      mbt_set_error("cannot open %s", conf_path);
+      return MBT_EIO;
+    }
+    ok = read_inp(fp, c);
+    fclose(fp);
+  } else {
+    conf_text T;
+    if (conf_load(conf_path, &T) != 0) {
+      fprintf(stderr, "ERROR: Cannot open file %s for reading.\n", conf_path);
+      mbt_set_error("cannot open %s", conf_path);
+      return MBT_EIO;
+    }
+    ok = read_conf(&T, c);
+    conf_free(&T);
   }
-  int ok = read_conf(&T, c);
-  conf_free(&T);
   if (!ok) {
     if (!g_err[0]) mbt_set_error("read_input failed for %s", conf_path);
     return MBT_EINVAL;

@@ -69,6 +69,21 @@ def make(name):
 
 
 
+def make_inp():
+    """Derived parameters of the reference's read_input on the legacy .inp form
+    (tests/variants.py INP_LINES), from the harness with 0 turns."""
+    from variants import write_inp
+    with tempfile.TemporaryDirectory() as d:
+        path = write_inp(d)
+        r = subprocess.run([HARNESS, path, d, "0"], capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        pn, pv = load_params(os.path.join(d, "params.txt"))
+    out = os.path.join(HERE, "inp_params.npz")
+    np.savez_compressed(out, param_names=pn, param_values=pv)
+    return out
+
+
 # per-variant section overrides of the end-to-end cases (none at present)
 E2E_OVERRIDES = {}
 E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform"]
@@ -177,9 +192,11 @@ def make_wofz():
 if __name__ == "__main__":
     from variants import WS_VARIANTS
     names = sys.argv[1:] or (sorted(VARIANTS) + ["e2e:" + c for c in E2E_CASES] + sorted(WS_VARIANTS)
-                             + ["wofz"])
+                             + ["wofz", "inp"])
     for n in names:
-        if n.startswith("e2e:"):
+        if n == "inp":
+            print(n, make_inp())
+        elif n.startswith("e2e:"):
             print(n, make_e2e(n[4:]))
         elif n in WS_VARIANTS:
             print(n, make_ws(n))

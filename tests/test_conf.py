@@ -83,3 +83,39 @@ def test_unsupported_and_missing(tmp_path):
     p.write_text("[tracking]\n jobtitle = t\n BunchModel = WEAK\n TrackPlane_LON = 0\n")
     with pytest.raises(m.MbtError):
         m.Config(str(p))
+
+
+def test_legacy_inp_params_bitwise(tmp_path):
+    """The legacy fixed-order input (read_input_file, src/input.c:198-276, comment
+    lines starting with C/c): every derived parameter equals the reference's,
+    from its own read_input + setup_* on the same file (tests/golden/inp_params.npz,
+    made by oracle/_ref/refharness; the harness and this reader both set
+    NrevScan = NrevTot + 1, which the reference's .inp path leaves 0).  The
+    reference's .inp path never sets U0 and taue, so Urad is 0 and the damping
+    constants are infinite there and here alike."""
+    import numpy as np
+    from variants import write_inp
+    g = np.load(os.path.join(os.path.dirname(INPUT), "inp_params.npz"))
+    cfg = m.Config(write_inp(str(tmp_path)), str(tmp_path) + "/")
+    dump = str(tmp_path / "mine.txt")
+    cfg.dump(dump)
+    mine, outs = {}, []
+    for line in open(dump):
+        p = line.split()
+        if len(p) == 2 and p[0] == "bunch_out":
+            outs.append(int(p[1]))
+        elif len(p) == 2:
+            try:
+                mine[p[0]] = float(p[1])
+            except ValueError:
+                pass
+    checked = 0
+    for k, v in zip(g["param_names"], g["param_values"]):
+        k = str(k)
+        if k == "bunch_out" or k not in mine:
+            continue
+        assert mine[k] == v or (np.isnan(v) and np.isnan(mine[k])), (k, mine[k], v)
+        checked += 1
+    assert checked > 60
+    assert outs == [int(v) for k, v in zip(g["param_names"], g["param_values"]) if str(k) == "bunch_out"]
+    assert cfg.get("Gzix") == 1.0 and cfg.geti("Np") == 1000 and cfg.geti("Nmlt") == 19

@@ -158,3 +158,39 @@ def qe_stat_compare(a, b):
     w = slice(-(len(a) // 3), None)
     eq = np.abs(a[w].mean(0) - b[w].mean(0)) / a[w].mean(0)
     return eq, (np.abs(a - b) / a).max(0)
+
+
+# ---- the legacy fixed-order input (read_input_file, src/input.c:198-276): the
+# benchmark ring's values in its 21 data lines, with comment lines between
+INP_LINES = [
+    ("C job title", "mtrack-mi355x legacy input"),
+    ("C E0 [GeV]", "3.0"),
+    ("C Lc [m]  rho0 [m]", "354.0 8.0"),
+    ("C betaH betaV dispH", "10.0 10.0 0.0"),
+    ("C alphaH alphaV disppH", "0.0 0.0 0.0"),
+    ("C emittanceH couplbeta", "1.0 0.01"),
+    ("C ac Je", "3.0e-4 2.0"),
+    ("C Vrf0 [MV] frf [MHz]", "5.53 352.3"),
+    ("C Iring [mA]", "208.0"),
+    ("C ApertureH ApertureV", "20.0 10.0"),
+    ("C chromaticity H V", "1.0 1.0"),
+    ("C TrackPlane LON HOR VER", "1 1 1"),
+    ("C NrevTot Nmlt NrevMon", "100 19 10"),
+    ("C Np", "1000"),
+    ("C nGen iseed per plane", "1 1001 1 2002 1 3003"),
+    ("C distribution files", "Ldist"),
+    ("c", "Hdist"),
+    ("c", "Vdist"),
+    ("C tau: CM offset, eps: CM offset, sgm_tau, sgm_eps", "0.0 0.0 0.010007 1.0e-3"),
+    ("C x", "0.01 0.0 0.1 0.01"),
+    ("C z", "0.001 0.0 0.01 0.001"),
+]
+
+
+def write_inp(d, name="legacy"):
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, name + ".inp")
+    with open(path, "w") as f:
+        for comment, data in INP_LINES:
+            f.write(comment + "\n" + data + "\n")
+    return path
