@@ -197,7 +197,8 @@ typedef struct mbt_ring mbt_ring;
                                    HIP graphs of whole blocks of turns */
 #define MBT_RING_NO_RW_GEMM 4   /* ensembles: keep the per-bunch resistive-wall sum even where the
                                    rings share one tap table (otherwise one FP64 MFMA GEMM) */
-#define MBT_RING_MIXED 8        /* mixed precision: wake (selffield) convolutions in FP32 and the
+#define MBT_RING_MIXED 8        /* mixed precision: wake (selffield) convolutions in FP32 with
+                                   hi/lo compensation (double-float sums) and the
                                    resistive-wall sum as an FP32 MFMA GEMM with hi/lo operand
                                    splitting and FP64 sums of FP32 chunks (any number of rings
                                    sharing a tap table); coordinates, phase advance and the

@@ -85,7 +85,7 @@
 struct DevParams {
   int ring, Nb, h, Nc, Nmlt;
   unsigned long long zero_bits; // always 0: the timing pass's stores xor with it
-  int mixed;                    // FP32 wake convolutions (MBT_RING_MIXED)
+  int mixed;                    // compensated FP32 wake convolutions (MBT_RING_MIXED)
   int bunch_off;  // first global bunch of the ring
   int bucket_off; // first global bucket of the ring
   long long rw_off, st_off; // offsets into the RW history and sqrt table
@@ -661,6 +661,27 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
 
 // ---------------------------------------------------------------- k_reduce
 
+// Wake convolution term by term in FP32 with hi/lo compensation (mixed
+// precision): each FP64 operand split into a high and a low float, the product
+// of the high parts exact through fmaf, the cross terms added to the low part,
+// and the running sum kept as an unevaluated float pair (two-sum), so the
+// cancellations of an oscillating Green function do not amplify FP32 rounding.
+struct F2 {
+  float hi, lo;
+};
+__device__ __forceinline__ void f2_add_prod(F2 &s, double a, double b)
+{
+  const float ah = (float)a, al = (float)(a - (double)ah);
+  const float bh = (float)b, bl = (float)(b - (double)bh);
+  const float p = ah * bh;
+  const float pe = fmaf(ah, bh, -p) + (ah * bl + al * bh);
+  const float t = s.hi + p;
+  const float bb = t - s.hi;
+  const float e = (s.hi - (t - bb)) + (p - bb);
+  s.hi = t;
+  s.lo = s.lo + (e + pe);
+}
+
 struct RedOut {
   double *stats;     // [STATS_TURNS][G][12]
   double *amp;       // [STATS_TURNS][G][6]
@@ -868,9 +889,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     if (P.convL && i >= icmin) {
       double Gn = 0.0;
       if (P.mixed) {
-        float g = 0.0f;
-        for (int k = i; k >= icmin; k--) g += (float)fnp[k] * (float)Gl[i - k];
-        Gn = (double)g;
+        F2 g = {0.0f, 0.0f};
+        for (int k = i; k >= icmin; k--) f2_add_prod(g, fnp[k], Gl[i - k]);
+        Gn = (double)g.hi + (double)g.lo;
       } else {
         for (int k = i; k >= icmin; k--) Gn += fnp[k] * Gl[i - k];
       }
@@ -892,9 +913,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     if (P.kickV) {
       double Gn = 0.0;
       if (P.mixed) {
-        float g = 0.0f;
-        for (int k = i; k >= 0; k--) g += (float)dV[k] * (float)Gv[i - k];
-        Gn = (double)g;
+        F2 g = {0.0f, 0.0f};
+        for (int k = i; k >= 0; k--) f2_add_prod(g, dV[k], Gv[i - k]);
+        Gn = (double)g.hi + (double)g.lo;
       } else {
         for (int k = i; k >= 0; k--) Gn += dV[k] * Gv[i - k];
       }
@@ -903,9 +924,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     if (P.kickH && i >= 1) {
       double Gn = 0.0;
       if (P.mixed) {
-        float g = 0.0f;
-        for (int k = i; k >= 0; k--) g += (float)dH[k] * (float)Gh[i - k];
-        Gn = (double)g;
+        F2 g = {0.0f, 0.0f};
+        for (int k = i; k >= 0; k--) f2_add_prod(g, dH[k], Gh[i - k]);
+        Gn = (double)g.hi + (double)g.lo;
       } else {
         for (int k = i; k >= 0; k--) Gn += dH[k] * Gh[i - k];
       }
