@@ -159,8 +159,9 @@ def c1(turns=2000):
     out["steps"] = turns
     out["config"] = {"workload": "64 bunches (buckets int(6.5 k) of h 416) x 1,000 macroparticles, QE on, "
                                  "copper RW 12 mm over 19 turns, 10 GHz broadband resonator in 50 slices, 208 mA",
-                     "precision": "mixed: FP32 wake convolutions, FP32 hi/lo MFMA RW GEMM with FP64 chunk sums; "
-                                  "coordinates, phase advance and RF recurrence FP64"}
+                     "precision": "mixed: hi/lo-compensated FP32 wake convolutions (a lone ring keeps its "
+                                  "latency-bound per-bunch FP64 RW sum; ensembles use the FP32 hi/lo MFMA RW "
+                                  "GEMM); coordinates, phase advance and RF recurrence FP64"}
     return out
 
 

@@ -199,9 +199,10 @@ typedef struct mbt_ring mbt_ring;
                                    rings share one tap table (otherwise one FP64 MFMA GEMM) */
 #define MBT_RING_MIXED 8        /* mixed precision: wake (selffield) convolutions in FP32 with
                                    hi/lo compensation (double-float sums) and the
-                                   resistive-wall sum as an FP32 MFMA GEMM with hi/lo operand
-                                   splitting and FP64 sums of FP32 chunks (any number of rings
-                                   sharing a tap table); coordinates, phase advance and the
+                                   resistive-wall GEMM of ensembles (where the FP64 build uses
+                                   one) in FP32 MFMA with hi/lo operand splitting and FP64 sums
+                                   of FP32 chunks; a lone ring keeps its per-bunch FP64 RW sum,
+                                   which is latency-bound; coordinates, phase advance and the
                                    LR / RF-feedback recurrence stay FP64 */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);

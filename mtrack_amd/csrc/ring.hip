@@ -1668,7 +1668,8 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
 {
   if (!r) return;
   (void)hipSetDevice(r->device);
-  void *ptrs[] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp, r->wg_first, r->wg_end, r->bunch_first,
+  // the six coordinate arrays are one allocation starting at C.tau
+  void *ptrs[] = {r->C.tau, r->wg_first, r->wg_end, r->bunch_first,
                   r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
@@ -1817,7 +1818,7 @@ static int setup_rw_gemm(mbt_ring *r)
       return MBT_OK; // rings with different taps: the per-bunch sum
     nrw++;
   }
-  if (nrw < (r->mixed ? 1 : RWG_MIN_RINGS)) return MBT_OK;
+  if (nrw < RWG_MIN_RINGS) return MBT_OK;
   RWGemm &g = r->rwg;
   g.h = h;
   g.M = (Nmlt + 1) * h;
@@ -1965,12 +1966,14 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
       }
     }
   }
-  DA(r->C.tau, Nalloc);
-  DA(r->C.del, Nalloc);
-  DA(r->C.x, Nalloc);
-  DA(r->C.xp, Nalloc);
-  DA(r->C.z, Nalloc);
-  DA(r->C.zp, Nalloc);
+  // one allocation for the six coordinate arrays (each Nalloc long, a multiple
+  // of 32 doubles, so every array starts on a 256-byte boundary)
+  DA(r->C.tau, 6 * (size_t)Nalloc);
+  r->C.del = r->C.tau + Nalloc;
+  r->C.x = r->C.del + Nalloc;
+  r->C.xp = r->C.x + Nalloc;
+  r->C.z = r->C.xp + Nalloc;
+  r->C.zp = r->C.z + Nalloc;
   DA(r->wg_first, nwg);
   DA(r->wg_end, nwg);
   DA(r->wg_bunch, nwg);
