@@ -65,6 +65,20 @@ VARIANTS = {
     "rsis": dict(sections={"selffield": {"Ncell": 60, "Nsigma": 9.0, "RsisL": 300.0, "aindL": 5.0},
                            "tracking": {"NrevTot": 30}, "ring": _ring(), "distribution": {"Np": 400}},
                  extra_sections=[BBR_L]),
+    # edge: one bunch, an odd macroparticle count below one work-item chunk, six
+    # slices over +-1.5 sigma (particles in both trash counters), RW over one turn
+    "edge_single": dict(sections={"selffield": {"Ncell": 6, "Nsigma": 1.5},
+                                  "tracking": {"NrevTot": 25, "filling": "SINGLE", "EnableRW_long": 1, "Nmlt": 1},
+                                  "ring": _ring(Iring=5.0), "distribution": {"Np": 37}},
+                        extra_sections=[BBR_L]),
+    # edge: every bucket filled with an odd count, bins over +-1 sigma only (most
+    # particles clamped into the edge bins by the selffield rule and dropped by
+    # the long-range rule), a long-range resonator reading those histograms
+    "edge_trash": dict(sections={"selffield": {"Ncell": 10, "Nsigma": 1.0},
+                                 "tracking": {"NrevTot": 25},
+                                 "ring": _ring(), "distribution": {"Np": 999}},
+                       extra_sections=[BBR_L, ("longrange_resonator_1", {"plane": "LON", "fres": 352.5, "Rs": 1e6,
+                                                                         "Qfactor": 3e3})]),
 }
 
 FILL_BUCKETS = [int(k * 6.5) for k in range(64)]
