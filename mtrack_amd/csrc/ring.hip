@@ -1078,131 +1078,135 @@ __global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, con
 // a Toeplitz tap matrix times one column per (ring, plane) of
 //   Y[a][col] = hist[(-rev h + a) mod n][plane] * Ibunch[a mod h]   (filled buckets)
 // -- the sum of src/transform_weak.c:806-824 for every bunch of every ring; the
-// ring's fNp and RWconst scale the result.  FP64 MFMA (16x16x4), split-K over
-// the four waves of a block.
+// ring's fNp and RWconst scale the result.
+//
+// k_rwg_tile: one block per (16-column tile, K slice); each wave takes up to
+// RWG_RT row tiles per pass, forms its history element of Y on the fly once per
+// K step and feeds it to all of them (MFMA 16x16x4: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]); the block writes its K slice's partial sums.
+// k_rwg_sum adds the slices in a fixed order and stores the pending kicks.
+// FP64: v_mfma_f64_16x16x4_f64.  F32 (mixed precision): both operands split into
+// high and low floats, hi*hi + hi*lo + lo*hi with v_mfma_f32_16x16x4_f32, summed
+// in FP32 over RWG32_CHUNK steps and then in FP64.
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+#define RWG_KS 16
+#define RWG_RT 8
+#define RWG32_CHUNK 64
 
 struct RWGemm {
   int K;        // rows of Y (a = 0 .. K-1), multiple of 16
   int N;        // columns: 2 per RW ring
   int M;        // taps (Nmlt + 1) h
   int h, n;     // buckets, history length (Nmlt + 2) h
+  int HP;       // h rounded up to 16 (rows of a partial)
   const int *col_ring;     // [N/2] ring of each column pair
   const double *c;         // [M + 1] tap weights, c[0] unused
-  double *Y;               // [K][N]
+  double *part;            // [RWG_KS][HP][N] K-slice partial sums
   const int *bucket_bunch; // [H] global bucket -> bunch (-1 empty)
 };
 
-__global__ __launch_bounds__(256) void k_rwg_y(RWGemm Gm, const DevParams *__restrict__ Pr,
-                                             const TurnState *__restrict__ tsr, const double *rwhist,
-                                             const double *Ibunch, const int *nfFill)
+template <bool F32>
+__global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                                const TurnState *__restrict__ tsr, const double *rwhist,
+                                                const double *Ibunch, const int *nfFill)
+{
+  const int tj = blockIdx.x, ks = blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kq = lane >> 4, l16 = lane & 15;
+  const int j = tj * 16 + l16;
+  const bool jv = j < Gm.N;
+  const int ring = Gm.col_ring[(jv ? j : 0) >> 1], pl = j & 1;
+  const DevParams &P = Pr[ring];
+  const long long rev = tsr[ring].rev;
+  const double *hist = rwhist + P.rw_off;
+  const double *Ib = Ibunch + P.bucket_off;
+  const int *fill = nfFill + P.bucket_off;
+  const int h = Gm.h, M = Gm.M, n = Gm.n;
+  const int ksteps = Gm.K / 4, per = (ksteps + RWG_KS - 1) / RWG_KS;
+  const int s0 = ks * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
+  const int nrt = Gm.HP / 16;
+  for (int rt0 = wv; rt0 < nrt; rt0 += 4 * RWG_RT) {
+    mfma_d4 acc[RWG_RT];
+    mfma_f4 acc32[RWG_RT];
+#pragma unroll
+    for (int g = 0; g < RWG_RT; g++) {
+      acc[g] = (mfma_d4){0.0, 0.0, 0.0, 0.0};
+      acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    // this lane's history element at a = 4 st + kq, advanced by 4 per step
+    int a = 4 * s0 + kq;
+    int q = a % h;
+    long long i0 = (-rev * h + a) % n;
+    int idx = (int)(i0 < 0 ? i0 + n : i0);
+    for (int st = s0; st < s1; st++) {
+      double bv = 0.0;
+      if (jv && a >= 1 && a < h + M && fill[q]) bv = hist[2 * idx + pl] * Ib[q];
+      const float bh = (float)bv, bl = (float)(bv - (double)bh);
+#pragma unroll
+      for (int g = 0; g < RWG_RT; g++) {
+        const int rt = rt0 + 4 * g;
+        if (rt < nrt) {
+          const int i = rt * 16 + l16, m = a - i;
+          const double av = (i < h && m >= 1 && m <= M) ? Gm.c[m] : 0.0;
+          if (F32) {
+            const float ah = (float)av, al = (float)(av - (double)ah);
+            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, acc32[g], 0, 0, 0);
+            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc32[g], 0, 0, 0);
+            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc32[g], 0, 0, 0);
+          } else {
+            acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[g], 0, 0, 0);
+          }
+        }
+      }
+      if (F32 && ((st - s0 + 1) % RWG32_CHUNK == 0 || st == s1 - 1)) {
+#pragma unroll
+        for (int g = 0; g < RWG_RT; g++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[g][v] += (double)acc32[g][v];
+          acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+      }
+      a += 4;
+      q += 4;
+      while (q >= h) q -= h;
+      idx += 4;
+      while (idx >= n) idx -= n;
+    }
+    if (jv) {
+#pragma unroll
+      for (int g = 0; g < RWG_RT; g++) {
+        const int rt = rt0 + 4 * g;
+        if (rt >= nrt) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          // result layouts: f64 16x16x4 element v of lane l is row l / 16 + 4 v,
+          // f32 16x16x4 row 4 (l / 16) + v; column l % 16 in both
+          const int row = rt * 16 + (F32 ? 4 * kq + v : kq + 4 * v);
+          Gm.part[((size_t)ks * Gm.HP + row) * Gm.N + j] = acc[g][v];
+        }
+      }
+    }
+  }
+}
+
+// one lane per (bucket, column): the K slices summed in a fixed order, then the
+// pending kick of the bucket's bunch (the ring's fNp and RW constant applied)
+__global__ __launch_bounds__(256) void k_rwg_sum(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                               const TurnState *__restrict__ tsr, double *pending)
 {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)Gm.K * Gm.N) return;
-  const int a = (int)(e / Gm.N), col = (int)(e % Gm.N);
+  if (e >= (long long)Gm.h * Gm.N) return;
+  const int kb = (int)(e / Gm.N), col = (int)(e % Gm.N);
   const int ring = Gm.col_ring[col >> 1], pl = col & 1;
   const DevParams &P = Pr[ring];
-  double v = 0.0;
-  if (a >= 1 && a < Gm.h + Gm.M) {
-    const int q = a % Gm.h;
-    if (nfFill[P.bucket_off + q]) {
-      const long long rev = tsr[ring].rev;
-      long long idx = (-rev * Gm.h + a) % Gm.n;
-      if (idx < 0) idx += Gm.n;
-      v = rwhist[P.rw_off + 2 * idx + pl] * Ibunch[P.bucket_off + q];
-    }
-  }
-  Gm.Y[e] = v;
-}
-
-// Sums the four waves' tiles and writes the pending kicks of the tile's bunches
-// (the ring's fNp and RW constant applied).
-__device__ __forceinline__ void rwg_store(const RWGemm &Gm, const DevParams *__restrict__ Pr,
-                                          const TurnState *__restrict__ tsr, double *pending,
-                                          double (*acc_s)[16][17], int ti, int tj)
-{
-  __syncthreads();
-  const int r = threadIdx.x >> 4, cc = threadIdx.x & 15;
-  const double sum = ((acc_s[0][r][cc] + acc_s[1][r][cc]) + acc_s[2][r][cc]) + acc_s[3][r][cc];
-  const int kb = ti * 16 + r, col = tj * 16 + cc;
-  if (kb < Gm.h && col < Gm.N) {
-    const int ring = Gm.col_ring[col >> 1], pl = col & 1;
-    const DevParams &P = Pr[ring];
-    const int g = Gm.bucket_bunch[P.bucket_off + kb];
-    if (g >= 0) {
-      const int trig = tsr[ring].m < 50 ? 1 : 0;
-      const bool on = pl == 0 ? P.track_h : P.track_v;
-      pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
-    }
-  }
-}
-
-// one block (4 waves) per 16 x 16 output tile; wave w covers a quarter of K
-__global__ __launch_bounds__(256) void k_rwg_mfma(RWGemm Gm, const DevParams *__restrict__ Pr,
-                                                const TurnState *__restrict__ tsr, double *pending)
-{
-  __shared__ double acc_s[4][16][17];
-  const int ti = blockIdx.x, tj = blockIdx.y; // kb tile, column tile
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i = ti * 16 + (lane & 15); // A row (bucket kb) of this lane
-  const int kq = lane >> 4;            // k within the 4-step
-  const int j = tj * 16 + (lane & 15); // B column of this lane
-  const int ksteps = Gm.K / 4, per = (ksteps + 3) / 4;
-  const int s0 = wv * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
-  mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int st = s0; st < s1; st++) {
-    const int a = st * 4 + kq;
-    const int m = a - i;
-    const double av = (i < Gm.h && m >= 1 && m <= Gm.M) ? Gm.c[m] : 0.0;
-    const double bv = j < Gm.N ? Gm.Y[(size_t)a * Gm.N + j] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-  }
-  // f64 16x16x4 result layout: element v of lane l is D[row = l / 16 + 4 v][col = l % 16]
-#pragma unroll
-  for (int v = 0; v < 4; v++) acc_s[wv][(lane >> 4) + 4 * v][lane & 15] = acc[v];
-  rwg_store(Gm, Pr, tsr, pending, acc_s, ti, tj);
-}
-
-// The same GEMM in FP32 MFMA (MBT_RING_MIXED): each operand split into a high
-// and a low float, three products hi*hi + hi*lo + lo*hi per step, summed in
-// FP32 over RWG32_CHUNK steps and then into an FP64 accumulator.
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-#define RWG32_CHUNK 64
-
-__global__ __launch_bounds__(256) void k_rwg_mfma32(RWGemm Gm, const DevParams *__restrict__ Pr,
-                                                  const TurnState *__restrict__ tsr, double *pending)
-{
-  __shared__ double acc_s[4][16][17];
-  const int ti = blockIdx.x, tj = blockIdx.y;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i = ti * 16 + (lane & 15);
-  const int kq = lane >> 4;
-  const int j = tj * 16 + (lane & 15);
-  const int ksteps = Gm.K / 4, per = (ksteps + 3) / 4;
-  const int s0 = wv * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
-  double accd[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int c0 = s0; c0 < s1; c0 += RWG32_CHUNK) {
-    const int c1 = c0 + RWG32_CHUNK < s1 ? c0 + RWG32_CHUNK : s1;
-    mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int st = c0; st < c1; st++) {
-      const int a = st * 4 + kq;
-      const int m = a - i;
-      const double av = (i < Gm.h && m >= 1 && m <= Gm.M) ? Gm.c[m] : 0.0;
-      const double bv = j < Gm.N ? Gm.Y[(size_t)a * Gm.N + j] : 0.0;
-      const float ah = (float)av, al = (float)(av - (double)ah);
-      const float bh = (float)bv, bl = (float)(bv - (double)bh);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) accd[v] += (double)acc[v];
-  }
-  // f32 16x16x4 result layout: element v of lane l is D[row = 4 (l / 16) + v][col = l % 16]
-#pragma unroll
-  for (int v = 0; v < 4; v++) acc_s[wv][4 * (lane >> 4) + v][lane & 15] = accd[v];
-  rwg_store(Gm, Pr, tsr, pending, acc_s, ti, tj);
+  const int g = Gm.bucket_bunch[P.bucket_off + kb];
+  if (g < 0) return;
+  double sum = 0.0;
+  for (int ks = 0; ks < RWG_KS; ks++) sum += Gm.part[((size_t)ks * Gm.HP + kb) * Gm.N + col];
+  const int trig = tsr[ring].m < 50 ? 1 : 0;
+  const bool on = pl == 0 ? P.track_h : P.track_v;
+  pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
 }
 
 // Turn end, one block per ring, for turns without a resistive-wall pass;
@@ -1489,7 +1493,7 @@ struct mbt_ring {
   int mixed;         // MBT_RING_MIXED: FP32 wake convolutions, FP32 hi/lo RW GEMM
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
-  double *rwg_c, *rwg_Y;
+  double *rwg_c, *rwg_part;
   int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
   int chunk_items;   // macroparticles per lane of one work-group's chunk
   int any_qe, any_dip, any_gen, nrev_mon;
@@ -1668,14 +1672,13 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
 {
   if (!r) return;
   (void)hipSetDevice(r->device);
-  // the six coordinate arrays are one allocation starting at C.tau
-  void *ptrs[] = {r->C.tau, r->wg_first, r->wg_end, r->bunch_first,
+  void *ptrs[] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp, r->wg_first, r->wg_end, r->bunch_first,
                   r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
                   r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
                   r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
-                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_Y};
+                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
     if (ptrs[i]) (void)hipFree(ptrs[i]);
   if (r->evpool) {
@@ -1824,6 +1827,7 @@ static int setup_rw_gemm(mbt_ring *r)
   g.M = (Nmlt + 1) * h;
   g.n = (Nmlt + 2) * h;
   g.K = ((h + g.M) + 15) / 16 * 16;
+  g.HP = (h + 15) / 16 * 16;
   g.N = 2 * nrw;
   int *cr = (int *)malloc(sizeof(int) * nrw), *bb = (int *)malloc(sizeof(int) * r->H);
   double *c = (double *)malloc(sizeof(double) * (g.M + 1));
@@ -1838,7 +1842,7 @@ static int setup_rw_gemm(mbt_ring *r)
   int rc = dalloc(&r->rwg_col_ring, nrw);
   if (!rc) rc = dalloc(&r->rwg_bucket_bunch, r->H);
   if (!rc) rc = dalloc(&r->rwg_c, (size_t)g.M + 1);
-  if (!rc) rc = dalloc(&r->rwg_Y, (size_t)g.K * g.N);
+  if (!rc) rc = dalloc(&r->rwg_part, (size_t)RWG_KS * g.HP * g.N);
   if (!rc && (hipMemcpy(r->rwg_col_ring, cr, sizeof(int) * nrw, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_bucket_bunch, bb, sizeof(int) * r->H, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_c, c, sizeof(double) * (g.M + 1), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1851,7 +1855,7 @@ static int setup_rw_gemm(mbt_ring *r)
   g.col_ring = r->rwg_col_ring;
   g.bucket_bunch = r->rwg_bucket_bunch;
   g.c = r->rwg_c;
-  g.Y = r->rwg_Y;
+  g.part = r->rwg_part;
   r->rwg_on = 1;
   return MBT_OK;
 }
@@ -1966,14 +1970,12 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
       }
     }
   }
-  // one allocation for the six coordinate arrays (each Nalloc long, a multiple
-  // of 32 doubles, so every array starts on a 256-byte boundary)
-  DA(r->C.tau, 6 * (size_t)Nalloc);
-  r->C.del = r->C.tau + Nalloc;
-  r->C.x = r->C.del + Nalloc;
-  r->C.xp = r->C.x + Nalloc;
-  r->C.z = r->C.xp + Nalloc;
-  r->C.zp = r->C.z + Nalloc;
+  DA(r->C.tau, Nalloc);
+  DA(r->C.del, Nalloc);
+  DA(r->C.x, Nalloc);
+  DA(r->C.xp, Nalloc);
+  DA(r->C.z, Nalloc);
+  DA(r->C.zp, Nalloc);
   DA(r->wg_first, nwg);
   DA(r->wg_end, nwg);
   DA(r->wg_bunch, nwg);
@@ -2411,15 +2413,16 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   if ((rc = launch_reduce(r, 2))) return rc;
   if (E) CK(hipEventRecord(E[2], r->stream));
   if (r->rwg_on) {
-    const long long tot = (long long)r->rwg.K * r->rwg.N;
-    hipLaunchKernelGGL(k_rwg_y, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
-                       r->rwhist, r->Ibunch, r->nfFill);
+    const dim3 tg((r->rwg.N + 15) / 16, RWG_KS);
     if (r->mixed)
-      hipLaunchKernelGGL(k_rwg_mfma32, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream,
-                         r->rwg, r->P, r->ts, r->pending);
+      hipLaunchKernelGGL(k_rwg_tile<true>, tg, dim3(256), 0, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
+                         r->nfFill);
     else
-      hipLaunchKernelGGL(k_rwg_mfma, dim3((r->rwg.h + 15) / 16, (r->rwg.N + 15) / 16), dim3(256), 0, r->stream,
-                         r->rwg, r->P, r->ts, r->pending);
+      hipLaunchKernelGGL(k_rwg_tile<false>, tg, dim3(256), 0, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
+                         r->nfFill);
+    const long long tot = (long long)r->rwg.h * r->rwg.N;
+    hipLaunchKernelGGL(k_rwg_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
+                       r->pending);
     CK(hipGetLastError());
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                        !lr_launch);
