@@ -1080,20 +1080,22 @@ __global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, con
 // -- the sum of src/transform_weak.c:806-824 for every bunch of every ring; the
 // ring's fNp and RWconst scale the result.
 //
-// k_rwg_tile: one block per (16-column tile, K slice); each wave takes up to
-// RWG_RT row tiles per pass, forms its history element of Y on the fly once per
-// K step and feeds it to all of them (MFMA 16x16x4: lane l holds A[l % 16][l / 16]
-// and B[l / 16][l % 16]); the block writes its K slice's partial sums.
-// k_rwg_sum adds the slices in a fixed order and stores the pending kicks.
+// k_rwg_tile: one block per (16-column tile, K slice of `per` steps of 4).  The
+// block first forms its slice of Y for its 16 columns from the rings' cyclic
+// dipole histories and bucket currents, and the tap weights its rows need, in
+// LDS; then each wave multiplies up to RWG_RT row tiles per pass out of LDS
+// (MFMA 16x16x4: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]) and the
+// block writes its K slice's partial sums.  k_rwg_sum adds the slices in a fixed
+// order and stores the pending kicks.
 // FP64: v_mfma_f64_16x16x4_f64.  F32 (mixed precision): both operands split into
 // high and low floats, hi*hi + hi*lo + lo*hi with v_mfma_f32_16x16x4_f32, summed
 // in FP32 over RWG32_CHUNK steps and then in FP64.
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-#define RWG_KS 16
 #define RWG_RT 8
 #define RWG32_CHUNK 64
+#define RWG_LDS_MAX (64 * 1024)
 
 struct RWGemm {
   int K;        // rows of Y (a = 0 .. K-1), multiple of 16
@@ -1101,31 +1103,54 @@ struct RWGemm {
   int M;        // taps (Nmlt + 1) h
   int h, n;     // buckets, history length (Nmlt + 2) h
   int HP;       // h rounded up to 16 (rows of a partial)
+  int per, KS;  // K steps (of 4) per slice, slices
   const int *col_ring;     // [N/2] ring of each column pair
   const double *c;         // [M + 1] tap weights, c[0] unused
-  double *part;            // [RWG_KS][HP][N] K-slice partial sums
+  double *part;            // [KS][HP][N] K-slice partial sums
   const int *bucket_bunch; // [H] global bucket -> bunch (-1 empty)
 };
+
+// LDS bytes of one k_rwg_tile block: Y slice [4 per][16] + taps [4 per + HP - 1]
+static inline size_t rwg_lds(int per, int HP) { return sizeof(double) * ((size_t)68 * per + HP); }
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__restrict__ Pr,
                                                 const TurnState *__restrict__ tsr, const double *rwhist,
                                                 const double *Ibunch, const int *nfFill)
 {
-  const int tj = blockIdx.x, ks = blockIdx.y;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kq = lane >> 4, l16 = lane & 15;
-  const int j = tj * 16 + l16;
-  const bool jv = j < Gm.N;
-  const int ring = Gm.col_ring[(jv ? j : 0) >> 1], pl = j & 1;
-  const DevParams &P = Pr[ring];
-  const long long rev = tsr[ring].rev;
-  const double *hist = rwhist + P.rw_off;
-  const double *Ib = Ibunch + P.bucket_off;
-  const int *fill = nfFill + P.bucket_off;
-  const int h = Gm.h, M = Gm.M, n = Gm.n;
-  const int ksteps = Gm.K / 4, per = (ksteps + RWG_KS - 1) / RWG_KS;
-  const int s0 = ks * per, s1 = s0 + per < ksteps ? s0 + per : ksteps;
-  const int nrt = Gm.HP / 16;
+  extern __shared__ double sm[];
+  const int tj = blockIdx.x, ks = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, kq = lane >> 4, l16 = lane & 15;
+  const int h = Gm.h, M = Gm.M, HP = Gm.HP;
+  const int ksteps = Gm.K / 4;
+  const int s0 = ks * Gm.per, s1 = s0 + Gm.per < ksteps ? s0 + Gm.per : ksteps;
+  const int A0 = 4 * s0, nA = 4 * (s1 - s0);
+  double *Yf = sm;                        // [nA][16]
+  double *cs = sm + (size_t)64 * Gm.per;  // taps m = A0 - HP + 1 + e
+  // ---- stage the Y slice and the taps
+  for (int e = tid; e < nA * 16; e += 256) {
+    const int t = e >> 4, j = tj * 16 + (e & 15), a = A0 + t;
+    double v = 0.0;
+    if (j < Gm.N && a >= 1 && a < h + M) {
+      const int ring = Gm.col_ring[j >> 1];
+      const DevParams &P = Pr[ring];
+      const int q = a % h;
+      if (nfFill[P.bucket_off + q]) {
+        long long idx = (-tsr[ring].rev * h + a) % Gm.n;
+        if (idx < 0) idx += Gm.n;
+        v = rwhist[P.rw_off + 2 * idx + (j & 1)] * Ibunch[P.bucket_off + q];
+      }
+    }
+    Yf[e] = v;
+  }
+  const int mlo = A0 - HP + 1;
+  for (int e = tid; e < nA + HP - 1; e += 256) {
+    const int m = mlo + e;
+    cs[e] = (m >= 1 && m <= M) ? Gm.c[m] : 0.0;
+  }
+  __syncthreads();
+  // ---- row tiles out of LDS
+  const int nrt = HP / 16;
   for (int rt0 = wv; rt0 < nrt; rt0 += 4 * RWG_RT) {
     mfma_d4 acc[RWG_RT];
     mfma_f4 acc32[RWG_RT];
@@ -1134,24 +1159,20 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
       acc[g] = (mfma_d4){0.0, 0.0, 0.0, 0.0};
       acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
     }
-    // this lane's history element at a = 4 st + kq, advanced by 4 per step
-    int a = 4 * s0 + kq;
-    int q = a % h;
-    long long i0 = (-rev * h + a) % n;
-    int idx = (int)(i0 < 0 ? i0 + n : i0);
-    for (int st = s0; st < s1; st++) {
-      double bv = 0.0;
-      if (jv && a >= 1 && a < h + M && fill[q]) bv = hist[2 * idx + pl] * Ib[q];
+    for (int st = 0; st < nA / 4; st++) {
+      const int al = 4 * st + kq;
+      const double bv = Yf[al * 16 + l16];
       const float bh = (float)bv, bl = (float)(bv - (double)bh);
 #pragma unroll
       for (int g = 0; g < RWG_RT; g++) {
         const int rt = rt0 + 4 * g;
         if (rt < nrt) {
-          const int i = rt * 16 + l16, m = a - i;
-          const double av = (i < h && m >= 1 && m <= M) ? Gm.c[m] : 0.0;
+          const int i = rt * 16 + l16;
+          // A[i][a] = c[a - i]: index a - i - mlo = al - i + HP - 1; rows i >= h are zero
+          const double av = i < h ? cs[al - i + HP - 1] : 0.0;
           if (F32) {
-            const float ah = (float)av, al = (float)(av - (double)ah);
-            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, acc32[g], 0, 0, 0);
+            const float ah = (float)av, alo = (float)(av - (double)ah);
+            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(alo, bh, acc32[g], 0, 0, 0);
             acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc32[g], 0, 0, 0);
             acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc32[g], 0, 0, 0);
           } else {
@@ -1159,7 +1180,7 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
           }
         }
       }
-      if (F32 && ((st - s0 + 1) % RWG32_CHUNK == 0 || st == s1 - 1)) {
+      if (F32 && ((st + 1) % RWG32_CHUNK == 0 || st == nA / 4 - 1)) {
 #pragma unroll
         for (int g = 0; g < RWG_RT; g++) {
 #pragma unroll
@@ -1167,13 +1188,9 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
           acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
         }
       }
-      a += 4;
-      q += 4;
-      while (q >= h) q -= h;
-      idx += 4;
-      while (idx >= n) idx -= n;
     }
-    if (jv) {
+    const int j = tj * 16 + l16;
+    if (j < Gm.N) {
 #pragma unroll
       for (int g = 0; g < RWG_RT; g++) {
         const int rt = rt0 + 4 * g;
@@ -1183,7 +1200,7 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
           // result layouts: f64 16x16x4 element v of lane l is row l / 16 + 4 v,
           // f32 16x16x4 row 4 (l / 16) + v; column l % 16 in both
           const int row = rt * 16 + (F32 ? 4 * kq + v : kq + 4 * v);
-          Gm.part[((size_t)ks * Gm.HP + row) * Gm.N + j] = acc[g][v];
+          Gm.part[((size_t)ks * HP + row) * Gm.N + j] = acc[g][v];
         }
       }
     }
@@ -1203,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_rwg_sum(RWGemm Gm, const DevParams *__r
   const int g = Gm.bucket_bunch[P.bucket_off + kb];
   if (g < 0) return;
   double sum = 0.0;
-  for (int ks = 0; ks < RWG_KS; ks++) sum += Gm.part[((size_t)ks * Gm.HP + kb) * Gm.N + col];
+  for (int ks = 0; ks < Gm.KS; ks++) sum += Gm.part[((size_t)ks * Gm.HP + kb) * Gm.N + col];
   const int trig = tsr[ring].m < 50 ? 1 : 0;
   const bool on = pl == 0 ? P.track_h : P.track_v;
   pending[2 * g + pl] = on ? (P.fNp * sum) * P.rwconst[pl][trig] : 0.0;
@@ -1828,6 +1845,11 @@ static int setup_rw_gemm(mbt_ring *r)
   g.n = (Nmlt + 2) * h;
   g.K = ((h + g.M) + 15) / 16 * 16;
   g.HP = (h + 15) / 16 * 16;
+  // K slices of up to 96 steps, fitting the block's Y slice and taps in LDS
+  g.per = 96;
+  while (g.per > 4 && rwg_lds(g.per, g.HP) > RWG_LDS_MAX) g.per /= 2;
+  if (rwg_lds(g.per, g.HP) > RWG_LDS_MAX) return MBT_OK; // h too large: the per-bunch sum
+  g.KS = (g.K / 4 + g.per - 1) / g.per;
   g.N = 2 * nrw;
   int *cr = (int *)malloc(sizeof(int) * nrw), *bb = (int *)malloc(sizeof(int) * r->H);
   double *c = (double *)malloc(sizeof(double) * (g.M + 1));
@@ -1842,7 +1864,7 @@ static int setup_rw_gemm(mbt_ring *r)
   int rc = dalloc(&r->rwg_col_ring, nrw);
   if (!rc) rc = dalloc(&r->rwg_bucket_bunch, r->H);
   if (!rc) rc = dalloc(&r->rwg_c, (size_t)g.M + 1);
-  if (!rc) rc = dalloc(&r->rwg_part, (size_t)RWG_KS * g.HP * g.N);
+  if (!rc) rc = dalloc(&r->rwg_part, (size_t)g.KS * g.HP * g.N);
   if (!rc && (hipMemcpy(r->rwg_col_ring, cr, sizeof(int) * nrw, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_bucket_bunch, bb, sizeof(int) * r->H, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_c, c, sizeof(double) * (g.M + 1), hipMemcpyHostToDevice) != hipSuccess ||
@@ -2413,12 +2435,13 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   if ((rc = launch_reduce(r, 2))) return rc;
   if (E) CK(hipEventRecord(E[2], r->stream));
   if (r->rwg_on) {
-    const dim3 tg((r->rwg.N + 15) / 16, RWG_KS);
+    const dim3 tg((r->rwg.N + 15) / 16, r->rwg.KS);
+    const size_t lds = rwg_lds(r->rwg.per, r->rwg.HP);
     if (r->mixed)
-      hipLaunchKernelGGL(k_rwg_tile<true>, tg, dim3(256), 0, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
+      hipLaunchKernelGGL(k_rwg_tile<true>, tg, dim3(256), lds, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
                          r->nfFill);
     else
-      hipLaunchKernelGGL(k_rwg_tile<false>, tg, dim3(256), 0, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
+      hipLaunchKernelGGL(k_rwg_tile<false>, tg, dim3(256), lds, r->stream, r->rwg, r->P, r->ts, r->rwhist, r->Ibunch,
                          r->nfFill);
     const long long tot = (long long)r->rwg.h * r->rwg.N;
     hipLaunchKernelGGL(k_rwg_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,

@@ -80,6 +80,7 @@ struct WSP {
   int triggerRW;
   double Pref0, fNp, Dub0, Lc;
   double factBE, sxx, szz, deltT, aAx, aAz, aLx, aLz, den_dQ; // FBII fields
+  double iaAx, iaAz;                                          // 1 / aAx, 1 / aAz
   double factF, factjp0, CdeltT;
 };
 
@@ -154,11 +155,15 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 }
 
 // get_linearly_interpolated_Efields, src/fbi_weakstrong.c:454-580, on one
-// slice's 25 field values
-__device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double xx,
-                                          double yy, double &Ex, double &Ey)
+// slice's 25 field values.  The grid quotients multiply by the reciprocal
+// spacings (iax = 1 / aAx, iaz = 1 / aAz) in place of the reference's four FP64
+// divisions per interaction; such a quotient can differ from the divided one in
+// its last bit, which moves a cell choice only where the bilinear field is
+// continuous.
+__device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double iax,
+                                          double iaz, double xx, double yy, double &Ex, double &Ey)
 {
-  int k1x = (int)(xx / aAx), k1z = (int)(yy / aAz), k2x, k2z;
+  int k1x = (int)(xx * iax), k1z = (int)(yy * iaz), k2x, k2z;
   const bool pz = yy >= 0, px = xx >= 0;
   if (pz) {
     k2z = k1z < 2 ? k1z + 1 : k1z;
@@ -180,7 +185,7 @@ __device__ __forceinline__ void ws_interp(const double *ex, const double *ey, do
   }
   const int i1 = (k1x + 2) * 5 + k1z + 2, i2 = (k2x + 2) * 5 + k1z + 2;
   const int i3 = (k1x + 2) * 5 + k2z + 2, i4 = (k2x + 2) * 5 + k2z + 2;
-  const double fx = (xx - k1x * aAx) / aAx, fz = (yy - k1z * aAz) / aAz;
+  const double fx = (xx - k1x * aAx) * iax, fz = (yy - k1z * aAz) * iaz;
   double y5, y6, x5, x6;
   if (px) {
     y5 = ey[i1] + fx * (ey[i2] - ey[i1]);
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev
     double x = xx[cell] + msx * P.aLx, xd = 0.0, y = zz[cell] + msz * P.aLz, yd = 0.0;
     for (int c = cell + 1; c < S; c++) {
       double Ex, Ey;
-      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
       const double ax = P.factF * Ex;
       xd = xd + P.deltT * ax;
       x = x + P.deltT * xd;
@@ -457,7 +462,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
     double sx = 0.0, sy = 0.0;
     if (act) {
       double Ex, Ey;
-      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
       const double ax = P.factF * Ex;
       xd = xd + P.deltT * ax;
       x = x + P.deltT * xd;
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
         double sx = 0.0, sy = 0.0;
         if (act && c < S) {
           double Ex, Ey;
-          ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, x, y, Ex, Ey);
+          ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
           const double ax = P.factF * Ex;
           xd = xd + P.deltT * ax;
           x = x + P.deltT * xd;
@@ -870,6 +875,8 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
   mbt_ws_grid(c, &P.aLx, &P.aLz);
   P.aAx = 2.0 * P.aLx;
   P.aAz = 2.0 * P.aLz;
+  P.iaAx = 1.0 / P.aAx;
+  P.iaAz = 1.0 / P.aAz;
   P.den_dQ = 2 * K_EPS0 * sqrt(M_PI) * P.factBE * K_C * P.deltT;
   {
     double Pave = 2.7e-9, Pu20 = 5 * Pave, Pm = 0.15 * Pu20 / 1.3332, dm = 3.22e22 * Pm, sigmam = 1.78;
