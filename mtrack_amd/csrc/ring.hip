@@ -1080,10 +1080,10 @@ __global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, con
 // -- the sum of src/transform_weak.c:806-824 for every bunch of every ring; the
 // ring's fNp and RWconst scale the result.
 //
-// k_rwg_tile: one block per (16-column tile, K slice of `per` steps of 4).  The
-// block first forms its slice of Y for its 16 columns from the rings' cyclic
-// dipole histories and bucket currents, and the tap weights its rows need, in
-// LDS; then each wave multiplies up to RWG_RT row tiles per pass out of LDS
+// k_rwg_y forms Y (one lane per element, from the rings' cyclic dipole
+// histories and bucket currents).  k_rwg_tile: one block per (16-column tile,
+// K slice of `per` steps of 4); the block stages its slice of Y (rows of 16
+// consecutive doubles) and the tap weights its rows need in LDS, then each wave multiplies up to RWG_RT row tiles per pass out of LDS
 // (MFMA 16x16x4: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]) and the
 // block writes its K slice's partial sums.  k_rwg_sum adds the slices in a fixed
 // order and stores the pending kicks.
@@ -1106,12 +1106,34 @@ struct RWGemm {
   int per, KS;  // K steps (of 4) per slice, slices
   const int *col_ring;     // [N/2] ring of each column pair
   const double *c;         // [M + 1] tap weights, c[0] unused
+  double *Y;               // [K][N] history columns of this turn
   double *part;            // [KS][HP][N] K-slice partial sums
   const int *bucket_bunch; // [H] global bucket -> bunch (-1 empty)
 };
 
 // LDS bytes of one k_rwg_tile block: Y slice [4 per][16] + taps [4 per + HP - 1]
 static inline size_t rwg_lds(int per, int HP) { return sizeof(double) * ((size_t)68 * per + HP); }
+
+__global__ __launch_bounds__(256) void k_rwg_y(RWGemm Gm, const DevParams *__restrict__ Pr,
+                                             const TurnState *__restrict__ tsr, const double *rwhist,
+                                             const double *Ibunch, const int *nfFill)
+{
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)Gm.K * Gm.N) return;
+  const int a = (int)(e / Gm.N), col = (int)(e % Gm.N);
+  const int ring = Gm.col_ring[col >> 1], pl = col & 1;
+  const DevParams &P = Pr[ring];
+  double v = 0.0;
+  if (a >= 1 && a < Gm.h + Gm.M) {
+    const int q = a % Gm.h;
+    if (nfFill[P.bucket_off + q]) {
+      long long idx = (-tsr[ring].rev * Gm.h + a) % Gm.n;
+      if (idx < 0) idx += Gm.n;
+      v = rwhist[P.rw_off + 2 * idx + pl] * Ibunch[P.bucket_off + q];
+    }
+  }
+  Gm.Y[e] = v;
+}
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__restrict__ Pr,
@@ -1129,19 +1151,8 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
   double *cs = sm + (size_t)64 * Gm.per;  // taps m = A0 - HP + 1 + e
   // ---- stage the Y slice and the taps
   for (int e = tid; e < nA * 16; e += 256) {
-    const int t = e >> 4, j = tj * 16 + (e & 15), a = A0 + t;
-    double v = 0.0;
-    if (j < Gm.N && a >= 1 && a < h + M) {
-      const int ring = Gm.col_ring[j >> 1];
-      const DevParams &P = Pr[ring];
-      const int q = a % h;
-      if (nfFill[P.bucket_off + q]) {
-        long long idx = (-tsr[ring].rev * h + a) % Gm.n;
-        if (idx < 0) idx += Gm.n;
-        v = rwhist[P.rw_off + 2 * idx + (j & 1)] * Ibunch[P.bucket_off + q];
-      }
-    }
-    Yf[e] = v;
+    const int t = e >> 4, j = tj * 16 + (e & 15);
+    Yf[e] = j < Gm.N ? Gm.Y[(size_t)(A0 + t) * Gm.N + j] : 0.0;
   }
   const int mlo = A0 - HP + 1;
   for (int e = tid; e < nA + HP - 1; e += 256) {
@@ -1510,7 +1521,7 @@ struct mbt_ring {
   int mixed;         // MBT_RING_MIXED: FP32 wake convolutions, FP32 hi/lo RW GEMM
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
-  double *rwg_c, *rwg_part;
+  double *rwg_c, *rwg_part, *rwg_Y;
   int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
   int chunk_items;   // macroparticles per lane of one work-group's chunk
   int any_qe, any_dip, any_gen, nrev_mon;
@@ -1695,7 +1706,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
                   r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
                   r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
-                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part};
+                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part, r->rwg_Y};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
     if (ptrs[i]) (void)hipFree(ptrs[i]);
   if (r->evpool) {
@@ -1865,6 +1876,7 @@ static int setup_rw_gemm(mbt_ring *r)
   if (!rc) rc = dalloc(&r->rwg_bucket_bunch, r->H);
   if (!rc) rc = dalloc(&r->rwg_c, (size_t)g.M + 1);
   if (!rc) rc = dalloc(&r->rwg_part, (size_t)g.KS * g.HP * g.N);
+  if (!rc) rc = dalloc(&r->rwg_Y, (size_t)g.K * g.N);
   if (!rc && (hipMemcpy(r->rwg_col_ring, cr, sizeof(int) * nrw, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_bucket_bunch, bb, sizeof(int) * r->H, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_c, c, sizeof(double) * (g.M + 1), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1878,6 +1890,7 @@ static int setup_rw_gemm(mbt_ring *r)
   g.bucket_bunch = r->rwg_bucket_bunch;
   g.c = r->rwg_c;
   g.part = r->rwg_part;
+  g.Y = r->rwg_Y;
   r->rwg_on = 1;
   return MBT_OK;
 }
@@ -2435,6 +2448,9 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   if ((rc = launch_reduce(r, 2))) return rc;
   if (E) CK(hipEventRecord(E[2], r->stream));
   if (r->rwg_on) {
+    const long long ny = (long long)r->rwg.K * r->rwg.N;
+    hipLaunchKernelGGL(k_rwg_y, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
+                       r->rwhist, r->Ibunch, r->nfFill);
     const dim3 tg((r->rwg.N + 15) / 16, r->rwg.KS);
     const size_t lds = rwg_lds(r->rwg.per, r->rwg.HP);
     if (r->mixed)
