@@ -338,6 +338,7 @@ def main():
         cfg = m.Config(conf, d + "/")
         ring = m.Ring(cfg, graphs=not a.no_graphs)
         ring.prepare()
+        chunk, tune_ms = ring.autotune()
         nb, npb = ring.Nb, ring.Np
         if a.warmup > 0:
             ring.track(a.warmup)
@@ -372,7 +373,8 @@ def main():
         "data": "synthetic: reference Gaussian generator (glibc rand stream), seeded",
         "config": {"workload": "production ring FP64: 416 bunches x %d macroparticles, 100 slices, "
                                "broadband resonator + resistive wall 416 x 50 turns, QE off" % npb,
-                   "bunches": nb, "macroparticles_per_bunch": npb, "turns_per_step": 1},
+                   "bunches": nb, "macroparticles_per_bunch": npb, "turns_per_step": 1,
+                   "chunk_items": chunk, "autotune_ms": tune_ms},
         "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
