@@ -198,6 +198,76 @@ int mbt_generate_bunch(const mbt_config *c, int kb, double *out)
 /* ------------------------------------------------------------ Green functions */
 
 /* src/transform_weak.c:289-403 (broadband resonators) and :406-471 (wake files) */
+#include "rw_first_bin_data.h"
+
+/* First-bin value of the short-range resistive wall at u (RW_table_LON /
+ * RW_table_TRANS, src/rw_table.c:311-395): linear between the tabulated points
+ * u_k <= u < u_k+1, the last value beyond the table, 0 (an error) below it or at
+ * its last point. */
+static double rw_first_bin(const double *w, double u)
+{
+  if (u > 2000.) return w[RWFB_N - 1];
+  for (int k = 0; k < RWFB_N - 1; k++) {
+    const double d = rwfb_u[k] - u, dn = rwfb_u[k + 1] - u;
+    if (d <= 0. && dn > 0.) {
+      const double du = rwfb_u[k + 1] - rwfb_u[k], dw = w[k + 1] - w[k];
+      return w[k] - dw / du * d;
+    }
+  }
+  return 0.0;
+}
+
+/* construct_greensfunc_RW, src/transform_weak.c:474-547: the short-range
+ * resistive-wall wake added to the Green functions (1/t^(3/2) longitudinal and
+ * 1/sqrt(t) transverse tails from bin 1 on, tabulated first bins); the
+ * reference's operation order. */
+static int greens_rw(const mbt_config *c, double *Gl, double *Gv, double *Gh, int planes[3])
+{
+  const int Nc = c->Ncell;
+  const double sgmatau = c->sigma_tau, dTau = c->dT, pi = M_PI;
+  const double beffL1 = c->beffL[0], beffL2 = pow(c->beffL[1], 2);
+  const double beffV3 = pow(c->beffV[0], 3), beffH3 = pow(c->beffH[0], 3);
+  const double beffV4 = pow(c->beffV[1], 4), beffH4 = pow(c->beffH[1], 4);
+  const double sigmarw = 1.0 / c->rhorw, t_const = sgmatau * dTau;
+  if (c->TrackPlane[MBT_LON] && c->EnableRW_short_LON) {
+    const double ARW = sqrt(MBT_Z0 / (sigmarw * MBT_C_LIGHT * pi)) / (4 * beffL1 * pi) * c->Lc;
+    for (int nc = 1; nc < Nc; nc++) {
+      const double tRW = (nc + 0.5) * t_const;
+      Gl[nc] -= ARW / (tRW * sqrt(tRW));
+    }
+    const double s0 = pow((2 * beffL2 * c->rhorw / MBT_Z0), 1.0 / 3.0);
+    const double v = rw_first_bin(rwfb_lon, sgmatau * dTau * MBT_C_LIGHT / s0);
+    if (v == 0.0) {
+      mbt_set_error("short-range RW: bin width outside the first-bin table (longitudinal)");
+      return MBT_EINVAL;
+    }
+    Gl[0] += v * 4 * MBT_Z0 * MBT_C_LIGHT / (pi * beffL2) * c->Lc;
+    planes[MBT_LON]++;
+    if (!c->EnableRW_short) return MBT_OK;
+  }
+  for (int pl = MBT_HOR; pl <= MBT_VER; pl++) {
+    const int plane = pl == MBT_HOR ? MBT_VER : MBT_HOR; /* the reference does VER first */
+    if (!c->TrackPlane[plane]) continue;
+    const double b3 = plane == MBT_VER ? beffV3 : beffH3, b4 = plane == MBT_VER ? beffV4 : beffH4;
+    const double b1 = plane == MBT_VER ? c->beffV[1] : c->beffH[1];
+    double *G = plane == MBT_VER ? Gv : Gh;
+    const double ARW = sqrt(MBT_Z0 * MBT_C_LIGHT / sigmarw / pi) / (pi * b3) * c->Lc;
+    const double s0 = pow((2 * b1 * b1 * c->rhorw / MBT_Z0), 1.0 / 3.0);
+    const double v = rw_first_bin(rwfb_trans, sgmatau * dTau * MBT_C_LIGHT / s0);
+    if (v == 0.0) {
+      mbt_set_error("short-range RW: bin width outside the first-bin table (transverse)");
+      return MBT_EINVAL;
+    }
+    G[0] += v * 8 * MBT_Z0 * MBT_C_LIGHT / (pi * b4) * s0 * c->Lc;
+    for (int nc = 1; nc < Nc; nc++) {
+      const double tRW = (nc + 0.5) * t_const;
+      G[nc] += ARW / sqrt(tRW);
+    }
+    planes[plane]++;
+  }
+  return MBT_OK;
+}
+
 int mbt_greens_functions(const mbt_config *c, double *Gl, double *Gv, double *Gh, int planes[3])
 {
   const int Nc = c->Ncell;
@@ -293,10 +363,7 @@ int mbt_greens_functions(const mbt_config *c, double *Gl, double *Gv, double *Gh
     planes[pl]++;
     free(Gt);
   }
-  if (c->EnableRW_short || c->EnableRW_short_LON) {
-    mbt_set_error("short-range resistive-wall Green functions (EnableRW_short) are not supported");
-    return MBT_EUNSUPPORTED;
-  }
+  if (c->EnableRW_short || c->EnableRW_short_LON) return greens_rw(c, Gl, Gv, Gh, planes);
   return MBT_OK;
 }
 

@@ -42,6 +42,7 @@ def lib():
         L.orc_ring_track.argtypes = [vp, ctypes.c_long, dp, dp, dp]
         L.orc_ring_final_stats.argtypes = [vp, dp]
         L.orc_ring_histograms.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_ring_greens.argtypes = [vp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_generate_bunch.argtypes = [vp, ip, dp]
         L.orc_glibc_rand_seq.argtypes = [ctypes.c_uint, ip, ctypes.POINTER(ctypes.c_int)]
         L.orc_philox4x32.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint),
@@ -111,6 +112,13 @@ class OracleRing:
         st = np.zeros((self.Nb, 12))
         lib().orc_ring_final_stats(self.ptr, _dp(st))
         return st
+
+    def greens(self):
+        nc = max(self.Nc, 1)
+        g = [np.zeros(nc) for _ in range(3)]
+        planes = (ctypes.c_int * 3)()
+        lib().orc_ring_greens(self.ptr, _dp(g[0]), _dp(g[1]), _dp(g[2]), planes)
+        return g[0], g[1], g[2], list(planes)
 
     def histograms(self):
         c = np.zeros((self.Nb, max(self.Nc, 1)))

@@ -28,6 +28,8 @@ int orc_ring_prepare(orc_ring *r, double *init_stats);
 int orc_ring_track(orc_ring *r, long nturns, double *stats, double *ampinv, double *rf);
 int orc_ring_final_stats(orc_ring *r, double *stats);
 int orc_ring_histograms(const orc_ring *r, double *counts, int *trash);
+/* the Green functions Gl, Gv, Gh [Ncell] and the plane counters PlaneL/V/H */
+int orc_ring_greens(const orc_ring *r, double *Gl, double *Gv, double *Gh, int *planes);
 int orc_generate_bunch(const mbt_config *c, int kb, double *out);
 
 /* glibc rand() stream after srand(seed) */

@@ -86,7 +86,7 @@ def make_inp():
 
 # per-variant section overrides of the end-to-end cases (none at present)
 E2E_OVERRIDES = {}
-E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform"]
+E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform", "rw_short"]
 
 MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"

@@ -10,7 +10,9 @@ long-range kick: the reference reads beffH3/beffV3 only when EnableRW_short is
 set (src/input.c:465-478) and so divides by zero in every HOR/VER quantity
 (its files hold nan there), while this build reads them whenever the kick needs
 them; for that case a reference nan must face a number here and every other
-token must match.
+token must match.  "rw_short" sets EnableRW_short (and so the reads): the
+short-range resistive wall in all three planes with the long-range kick, every
+token compared.
 
 "ws_uniform" is the weak-strong (BunchModel = STRONG) manager with FBII and the
 resistive wall, with the reference's bunch currents; its files were written by
@@ -66,7 +68,7 @@ def compare_columns(a_text, b_text, rel):
     return ["column %d: %.3g > %g" % (c, e, rel) for c, e in enumerate(err) if e > rel]
 
 
-@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform"])
+@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform", "rw_short"])
 def test_cli_files_match_reference(tmp_path, case):
     gold = os.path.join(GOLD_ROOT, "e2e_" + case)
     d = str(tmp_path)

@@ -113,3 +113,20 @@ def test_fixtures_are_current(tmp_path, name):
     assert r.returncode == 0, r.stderr
     st = np.loadtxt(os.path.join(str(tmp_path), "stats.txt"), ndmin=2)
     assert np.array_equal(st, g["stats"])
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_greens_product_equals_oracle(tmp_path, name):
+    """The product's Green functions (mbt_greens_functions: resonators, short-range
+    resistive wall with its tabulated first bins) equal the oracle's bit for bit;
+    the oracle's are pinned to the reference through the bit-identical turns
+    above."""
+    path = write_variant(name, str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    if cfg.geti("Ncell") <= 0:
+        pytest.skip("no selffield")
+    gl, gv, gh, planes = cfg.greens()
+    orc = o.OracleRing(cfg)
+    ol, ov, oh, oplanes = orc.greens()
+    assert np.array_equal(gl, ol) and np.array_equal(gv, ov) and np.array_equal(gh, oh), name
+    assert planes == oplanes

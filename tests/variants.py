@@ -65,6 +65,13 @@ VARIANTS = {
     "rsis": dict(sections={"selffield": {"Ncell": 60, "Nsigma": 9.0, "RsisL": 300.0, "aindL": 5.0},
                            "tracking": {"NrevTot": 30}, "ring": _ring(), "distribution": {"Np": 400}},
                  extra_sections=[BBR_L]),
+    # short-range resistive wall in all planes (tabulated first bins, 1/t^(3/2)
+    # and 1/sqrt(t) tails) with the long-range RW; bin width 26 s0
+    "rw_short": dict(sections={"selffield": {"Ncell": 50, "Nsigma": 5.0},
+                               "tracking": {"NrevTot": 30, "EnableRW_short": 1, "EnableRW_short_LON": 1,
+                                            "EnableRW_long": 1, "Nmlt": 3},
+                               "ring": _ring(Iring=60.0, beffH4=0.012, beffV4=0.012),
+                               "distribution": {"Np": 400}}),
     # edge: one bunch, an odd macroparticle count below one work-item chunk, six
     # slices over +-1.5 sigma (particles in both trash counters), RW over one turn
     "edge_single": dict(sections={"selffield": {"Ncell": 6, "Nsigma": 1.5},
