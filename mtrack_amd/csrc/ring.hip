@@ -10,9 +10,10 @@
 //   k_lr_par  long-range resonator phasor recurrence over all buckets and the
 //             RF-feedback samples (construct_wake_phasor src/transform_weak.c:961-1048):
 //             linear, so one block per resonator with the buckets in parallel and
-//             one short chain over bucket start phasors; k_lr keeps the one-lane
-//             recurrence (bit-identical arithmetic to the reference) for
-//             wake_phasor_init and very large h; k_fb_solve (or the turn end)
+//             one short chain over bucket start phasors (also for the replayed
+//             turns of wake_phasor_init); k_lr keeps the one-lane recurrence
+//             (bit-identical arithmetic to the reference) for h > 1365;
+//             k_fb_solve (or the turn end)
 //             runs the RF-feedback solve (src/worker_weakweak.c:268-278);
 //   k_turn    one pass over every macroparticle: pending RW kick, selffield
 //             kick (src/transform_weak.c:697-731), one-turn map
@@ -1347,10 +1348,19 @@ __global__ void k_lr(int nres, const LRConst *lc, const DevParams *__restrict__ 
 // (again the recurrence's own operations) and writes the slice terms and the
 // RF-feedback samples.  Only the start phasors are rounded differently from
 // the sequential loop (by the closed form of phase B).  Dynamic LDS: 6 h doubles.
+__device__ __forceinline__ void fb_sample(double *fbh, int fb_len, long long c, double v0, double v1, double p0,
+                                          double p1)
+{
+  const double vr = v0 * p0 - v1 * p1, vi = v0 * p1 + v1 * p0;
+  const unsigned key = ((unsigned)c % (unsigned)fb_len) * 2;
+  fbh[key] = sqrt(vr * vr + vi * vi);
+  fbh[key + 1] = atan2(vr, vi);
+}
+
 __global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevParams *__restrict__ Pr,
                                                 const double *fnp_ring, long long plane_stride, const double *Ibunch,
                                                 const int *nfFill, int Nc, double *phasor_end, double *lrc,
-                                                double *fb_hist, TurnState *tsr)
+                                                double *fb_hist, TurnState *tsr, int init)
 {
   extern __shared__ double lsm[];
   __shared__ double Pn[2];
@@ -1367,6 +1377,10 @@ __global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevPara
   const int fb_len = P.fb_len;
   const bool fb = L.fb && ts->has_fb;
   const long long cnt0 = ts->fb_count;
+  // wake_phasor_init (init): the same turn replayed init_turns times from a
+  // zero phasor without slice terms (src/transform_weak.c:869-952)
+  const int nturn = init ? L.init_turns : 1;
+  if (nturn <= 0) return;
   double *Lr = lsm, *Li = Lr + h, *Sr = Li + h, *Si = Sr + h, *Ar = Si + h, *Ai = Ar + h;
   // ---- phase A
   for (int i = tid; i < h; i += nth) {
@@ -1395,30 +1409,50 @@ __global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevPara
     Pn[1] = b;
   }
   __syncthreads();
-  // ---- phase B, in the recurrence's bucket order (last bucket first)
+  // ---- phase B, in the recurrence's bucket order (last bucket first); when
+  // replaying (init) it also takes the feedback samples that survive in the
+  // sample ring, one lane in order
   if (tid == 0) {
-    double V0 = phasor_end[2 * r], V1 = phasor_end[2 * r + 1];
+    double V0 = init ? 0.0 : phasor_end[2 * r], V1 = init ? 0.0 : phasor_end[2 * r + 1];
+    double W0 = phasor_end[2 * r], W1 = phasor_end[2 * r + 1]; // what the first sample reads
     const double g0 = L.ge * L.gc, g1 = L.ge * L.gs, e0 = L.ee * L.ec, e1 = L.ee * L.es;
-    for (int m = 0; m < h; m++) {
-      const int i = h - m - 1;
-      Sr[i] = V0;
-      Si[i] = V1;
-      if (fill[i] == 1) {
-        const double w0 = (V0 * Pn[0] - V1 * Pn[1]) + Lr[i], w1 = (V0 * Pn[1] + V1 * Pn[0]) + Li[i];
-        V0 = w0 * g0 - w1 * g1;
-        V1 = w0 * g1 + w1 * g0;
-      } else {
-        const double n0 = V0 * e0 - V1 * e1, n1 = V0 * e1 + V1 * e0;
-        V0 = n0;
-        V1 = n1;
+    const long long keep_init = cnt0 + 2LL * h * nturn - fb_len;
+    long long c = cnt0;
+    for (int k = 0; k < nturn; k++)
+      for (int m = 0; m < h; m++) {
+        const int i = h - m - 1;
+        if (init && fb) {
+          if (c >= keep_init) fb_sample(fbh, fb_len, c, W0, W1, L.pb0, L.pb1);
+          c++;
+        }
+        Sr[i] = V0;
+        Si[i] = V1;
+        if (fill[i] == 1) {
+          const double w0 = (V0 * Pn[0] - V1 * Pn[1]) + Lr[i], w1 = (V0 * Pn[1] + V1 * Pn[0]) + Li[i];
+          V0 = w0 * g0 - w1 * g1;
+          V1 = w0 * g1 + w1 * g0;
+        } else {
+          const double n0 = V0 * e0 - V1 * e1, n1 = V0 * e1 + V1 * e0;
+          V0 = n0;
+          V1 = n1;
+        }
+        Ar[i] = V0;
+        Ai[i] = V1;
+        W0 = V0;
+        W1 = V1;
+        if (init && fb) {
+          if (c >= keep_init) fb_sample(fbh, fb_len, c, W0, W1, L.pa0, L.pa1);
+          c++;
+        }
       }
-      Ar[i] = V0;
-      Ai[i] = V1;
-    }
     phasor_end[2 * r] = V0;
     phasor_end[2 * r + 1] = V1;
   }
   __syncthreads();
+  if (init) {
+    if (tid == 0 && fb) ts->fb_count = cnt0 + 2LL * h * nturn;
+    return;
+  }
   // ---- phase C; of the 2 h feedback samples only the last fb_len survive in
   // the sample ring, so only those are written (no two lanes share a slot)
   const long long keep_from = cnt0 + 2LL * h - fb_len;
@@ -1439,18 +1473,8 @@ __global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevPara
     }
     if (fb) {
       const long long c = cnt0 + 2LL * (h - i - 1);
-      if (c >= keep_from) {
-        const double vr = Sr[i] * L.pb0 - Si[i] * L.pb1, vi = Sr[i] * L.pb1 + Si[i] * L.pb0;
-        const unsigned key = ((unsigned)c % (unsigned)fb_len) * 2;
-        fbh[key] = sqrt(vr * vr + vi * vi);
-        fbh[key + 1] = atan2(vr, vi);
-      }
-      if (c + 1 >= keep_from) {
-        const double vr = Ar[i] * L.pa0 - Ai[i] * L.pa1, vi = Ar[i] * L.pa1 + Ai[i] * L.pa0;
-        const unsigned key = ((unsigned)(c + 1) % (unsigned)fb_len) * 2;
-        fbh[key] = sqrt(vr * vr + vi * vi);
-        fbh[key + 1] = atan2(vr, vi);
-      }
+      if (c >= keep_from) fb_sample(fbh, fb_len, c, Sr[i], Si[i], L.pb0, L.pb1);
+      if (c + 1 >= keep_from) fb_sample(fbh, fb_len, c + 1, Ar[i], Ai[i], L.pa0, L.pa1);
     }
   }
   __syncthreads(); // every lane has read the sample counter
@@ -2346,9 +2370,9 @@ static int launch_lr(mbt_ring *r, int init)
   if (r->nres == 0) return MBT_OK;
   const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
   const size_t lds = sizeof(double) * 6 * (size_t)r->lr_hmax;
-  if (!init && lds <= LR_PAR_LDS_MAX)
+  if (lds <= LR_PAR_LDS_MAX)
     hipLaunchKernelGGL(k_lr_par, dim3(r->nres), dim3(256), lds, r->stream, r->lc, r->P, r->fnp_ring, stride,
-                       r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts);
+                       r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
   else
     hipLaunchKernelGGL(k_lr, dim3((r->nres + 63) / 64), dim3(64), 0, r->stream, r->nres, r->lc, r->P, r->fnp_ring,
                        stride, r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
