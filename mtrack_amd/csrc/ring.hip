@@ -51,6 +51,8 @@
 #define MIN_CHUNK_ITEMS 4
 #define RWG_MIN_RINGS 8
 #define AUTOTUNE_CANDIDATES 4
+#define AUTOTUNE_ROUNDS 3
+#define AUTOTUNE_REPS 2
 #define MAX_CHUNK_ITEMS 64
 #define LR_PREFETCH 16
 // dynamic LDS of the bucket-parallel LR recurrence (6 doubles per bucket): up
@@ -1097,6 +1099,9 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 #define RWG_RT 8
 #define RWG32_CHUNK 64
 #define RWG_LDS_MAX (64 * 1024)
+#ifndef RWG_PER
+#define RWG_PER 96 // K steps (of 4) per slice
+#endif
 
 struct RWGemm {
   int K;        // rows of Y (a = 0 .. K-1), multiple of 16
@@ -1880,8 +1885,8 @@ static int setup_rw_gemm(mbt_ring *r)
   g.n = (Nmlt + 2) * h;
   g.K = ((h + g.M) + 15) / 16 * 16;
   g.HP = (h + 15) / 16 * 16;
-  // K slices of up to 96 steps, fitting the block's Y slice and taps in LDS
-  g.per = 96;
+  // K slices of up to RWG_PER steps, fitting the block's Y slice and taps in LDS
+  g.per = RWG_PER;
   while (g.per > 4 && rwg_lds(g.per, g.HP) > RWG_LDS_MAX) g.per /= 2;
   if (rwg_lds(g.per, g.HP) > RWG_LDS_MAX) return MBT_OK; // h too large: the per-bunch sum
   g.KS = (g.K / 4 + g.per - 1) / g.per;
@@ -2724,9 +2729,9 @@ extern "C" int mbt_ring_set_chunk(mbt_ring *r, int items)
 extern "C" int mbt_ring_get_chunk(const mbt_ring *r) { return r->chunk_items; }
 
 // Chooses the chunk (macroparticles per lane of a work-group) by timing the
-// statistics pass of k_turn (it reads every coordinate, bins and reduces like
-// the turn pass, and changes no state but the histogram accumulators, cleared
-// afterwards) for each candidate; keeps the fastest.  The choice changes only
+// ring's k_turn variant in MODE_TUNE (the turn's work with the coordinates
+// written back unchanged; it changes no state but the histogram accumulators,
+// cleared afterwards) for each candidate; keeps the fastest.  The choice changes only
 // how the moment sums are grouped, i.e. results at the rounding level.
 extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
 {
@@ -2739,25 +2744,34 @@ extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
   // the turn's own variant, run as a timing pass (coordinates unchanged)
   const turn_kernel_t k = g_tune_kernels[(r->any_qe ? 4 : 0) + (r->any_dip ? 2 : 0) + (r->any_gen ? 1 : 0)];
   const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
+  // candidates interleaved over AUTOTUNE_ROUNDS rounds, each timed over
+  // AUTOTUNE_REPS passes after one untimed pass; a candidate's time is its best
+  // round (the least disturbed by clocks and other work)
   int best = r->chunk_items, rc = MBT_OK;
-  double best_ms = 1e300;
-  for (int c = 0; c < AUTOTUNE_CANDIDATES && rc == MBT_OK; c++) {
-    if ((rc = set_decomposition(r, cand[c]))) break;
-    float t = 0.0f;
-    for (int rep = 0; rep < 4; rep++) {
-      if (rep == 1) CK(hipEventRecord(e0, r->stream));
-      hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
-                         r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
-                         r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
-      CK(hipGetLastError());
+  double tbest[AUTOTUNE_CANDIDATES];
+  for (int c = 0; c < AUTOTUNE_CANDIDATES; c++) tbest[c] = 1e300;
+  for (int round = 0; round < AUTOTUNE_ROUNDS && rc == MBT_OK; round++)
+    for (int c = 0; c < AUTOTUNE_CANDIDATES && rc == MBT_OK; c++) {
+      if ((rc = set_decomposition(r, cand[c]))) break;
+      float t = 0.0f;
+      for (int rep = 0; rep <= AUTOTUNE_REPS; rep++) {
+        if (rep == 1) CK(hipEventRecord(e0, r->stream));
+        hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
+                           r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
+                           r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
+        CK(hipGetLastError());
+      }
+      CK(hipEventRecord(e1, r->stream));
+      CK(hipEventSynchronize(e1));
+      CK(hipEventElapsedTime(&t, e0, e1));
+      const double m = t / AUTOTUNE_REPS;
+      if (m < tbest[c]) tbest[c] = m;
     }
-    CK(hipEventRecord(e1, r->stream));
-    CK(hipEventSynchronize(e1));
-    CK(hipEventElapsedTime(&t, e0, e1));
-    const double m = t / 3.0;
-    if (ms) ms[c] = m;
-    if (m < best_ms) {
-      best_ms = m;
+  double best_ms = 1e300;
+  for (int c = 0; c < AUTOTUNE_CANDIDATES; c++) {
+    if (ms) ms[c] = tbest[c];
+    if (tbest[c] < best_ms) {
+      best_ms = tbest[c];
       best = cand[c];
     }
   }
