@@ -5,16 +5,8 @@
 // rings (one ring is an ensemble of one): per-ring constants sit in a DevParams
 // table indexed through bunch -> ring, so every launch covers all rings and
 // rings of uneven bunch and macroparticle counts share each kernel.  One turn
-// is four kernels on one stream:
+// is a few kernels on one stream (replayed as a HIP graph per block of turns):
 //
-//   k_lr_par  long-range resonator phasor recurrence over all buckets and the
-//             RF-feedback samples (construct_wake_phasor src/transform_weak.c:961-1048):
-//             linear, so one block per resonator with the buckets in parallel and
-//             one short chain over bucket start phasors (also for the replayed
-//             turns of wake_phasor_init); k_lr keeps the one-lane recurrence
-//             (bit-identical arithmetic to the reference) for h > 1365;
-//             k_fb_solve (or the turn end)
-//             runs the RF-feedback solve (src/worker_weakweak.c:268-278);
 //   k_turn    one pass over every macroparticle: pending RW kick, selffield
 //             kick (src/transform_weak.c:697-731), one-turn map
 //             (src/transform_weak.c:21-192), shifted moment sums and the next
@@ -24,7 +16,17 @@
 //             convolutions for the next turn (src/transform_weak.c:633-731);
 //   k_rw      per bunch: resistive-wall long-range sum over (Nmlt+1) h past
 //             buckets (src/transform_weak.c:780-833), applied as a pending kick
-//             at the start of the next pass; k_end advances the turn counter.
+//             at the start of the next pass, and the turn end; for ensembles of
+//             rings sharing one tap table the sum is one MFMA GEMM instead
+//             (k_rwg_y, k_rwg_tile, k_rwg_sum, then k_end);
+//   k_lr_par  with long-range resonators: the phasor recurrence over all buckets
+//             and the RF-feedback samples (construct_wake_phasor
+//             src/transform_weak.c:961-1048), linear, so one block per resonator
+//             with the buckets in parallel and one short chain over bucket start
+//             phasors (also for the replayed turns of wake_phasor_init); k_lr
+//             keeps the one-lane recurrence (the reference's arithmetic) for
+//             h > 1365; then k_lr_sum and k_fb_solve, the RF-feedback solve
+//             (src/worker_weakweak.c:268-278).
 //
 // Determinism: histogram dipole sums are accumulated as exact 128-bit fixed-point
 // integers (LDS then global integer atomics), moment sums by fixed-order
