@@ -1102,7 +1102,7 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 #define RWG32_CHUNK 64
 #define RWG_LDS_MAX (64 * 1024)
 #ifndef RWG_PER
-#define RWG_PER 96 // K steps (of 4) per slice
+#define RWG_PER 48 // K steps (of 4) per slice (48 measured best of 24, 48, 96)
 #endif
 
 struct RWGemm {
