@@ -163,48 +163,27 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 __device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double iax,
                                           double iaz, double xx, double yy, double &Ex, double &Ey)
 {
-  int k1x = (int)(xx * iax), k1z = (int)(yy * iaz), k2x, k2z;
+  int k1x = (int)(xx * iax), k1z = (int)(yy * iaz);
   const bool pz = yy >= 0, px = xx >= 0;
-  if (pz) {
-    k2z = k1z < 2 ? k1z + 1 : k1z;
-    k1z = k1z > 2 ? 2 : k1z;
-    k2z = k2z > 2 ? 2 : k2z;
-  } else {
-    k2z = k1z > -2 ? k1z - 1 : k1z;
-    k1z = k1z < -2 ? -2 : k1z;
-    k2z = k2z < -2 ? -2 : k2z;
-  }
-  if (px) {
-    k2x = k1x < 2 ? k1x + 1 : k1x;
-    k1x = k1x > 2 ? 2 : k1x;
-    k2x = k2x > 2 ? 2 : k2x;
-  } else {
-    k2x = k1x > -2 ? k1x - 1 : k1x;
-    k1x = k1x < -2 ? -2 : k1x;
-    k2x = k2x < -2 ? -2 : k2x;
-  }
+  // the reference's two sign cases as selects (no divergence): towards +2 for
+  // a non-negative coordinate, towards -2 otherwise
+  int k2z = pz ? (k1z < 2 ? k1z + 1 : k1z) : (k1z > -2 ? k1z - 1 : k1z);
+  k1z = pz ? (k1z > 2 ? 2 : k1z) : (k1z < -2 ? -2 : k1z);
+  k2z = pz ? (k2z > 2 ? 2 : k2z) : (k2z < -2 ? -2 : k2z);
+  int k2x = px ? (k1x < 2 ? k1x + 1 : k1x) : (k1x > -2 ? k1x - 1 : k1x);
+  k1x = px ? (k1x > 2 ? 2 : k1x) : (k1x < -2 ? -2 : k1x);
+  k2x = px ? (k2x > 2 ? 2 : k2x) : (k2x < -2 ? -2 : k2x);
   const int i1 = (k1x + 2) * 5 + k1z + 2, i2 = (k2x + 2) * 5 + k1z + 2;
   const int i3 = (k1x + 2) * 5 + k2z + 2, i4 = (k2x + 2) * 5 + k2z + 2;
   const double fx = (xx - k1x * aAx) * iax, fz = (yy - k1z * aAz) * iaz;
-  double y5, y6, x5, x6;
-  if (px) {
-    y5 = ey[i1] + fx * (ey[i2] - ey[i1]);
-    y6 = ey[i3] + fx * (ey[i4] - ey[i3]);
-    x5 = ex[i1] + fx * (ex[i2] - ex[i1]);
-    x6 = ex[i3] + fx * (ex[i4] - ex[i3]);
-  } else {
-    y5 = ey[i1] - fx * (ey[i2] - ey[i1]);
-    y6 = ey[i3] - fx * (ey[i4] - ey[i3]);
-    x5 = ex[i1] - fx * (ex[i2] - ex[i1]);
-    x6 = ex[i3] - fx * (ex[i4] - ex[i3]);
-  }
-  if (pz) {
-    Ey = y5 + fz * (y6 - y5);
-    Ex = x5 + fz * (x6 - x5);
-  } else {
-    Ey = y5 - fz * (y6 - y5);
-    Ex = x5 - fz * (x6 - x5);
-  }
+  // a - f d == a + (-f) d exactly, so the signed fraction reproduces both cases
+  const double sx = px ? fx : -fx, sz = pz ? fz : -fz;
+  const double y5 = ey[i1] + sx * (ey[i2] - ey[i1]);
+  const double y6 = ey[i3] + sx * (ey[i4] - ey[i3]);
+  const double x5 = ex[i1] + sx * (ex[i2] - ex[i1]);
+  const double x6 = ex[i3] + sx * (ex[i4] - ex[i3]);
+  Ey = y5 + sz * (y6 - y5);
+  Ex = x5 + sz * (x6 - x5);
 }
 
 // ------------------------------------------------------------------- phase 1
