@@ -165,7 +165,7 @@ def c1(turns=2000):
     return out
 
 
-def batched(steps, warmup, nrings, seed, chunk=0, mixed=False):
+def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
     """The batched ensemble: nrings independent rings (bunch and macroparticle
     counts log-uniform, currents and chromaticities uniform, RF beam-loading
     feedback on; mtrack_amd.confgen.wl_ensemble) tracked together, FP64 (or
@@ -201,6 +201,29 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False):
         t1 = time.perf_counter()
         ms, launches = ens.kernel_times()
         bytes_turn = ens.bytes_per_turn()
+        growth = None
+        if growth_turns > 0:
+            # per-ring growth rates of the centroid action over further turns,
+            # tracked in monitor-interval blocks keeping the monitor rows only
+            from mtrack_amd.analysis import growth_rates_monitor
+            import numpy as np
+            ens.set_timing(False)
+            mon = cfgs[0].geti("NrevMon")
+            rows, tmon = [], []
+            start = warmup + 2 * steps  # turns tracked so far
+            for blk in range(growth_turns // mon):
+                _, am, _ = ens.track(mon, want_stats=False, want_rf=False)
+                r = (mon - 1 - start % mon) % mon  # the block's monitor turn
+                rows.append(am[r].copy())
+                tmon.append(start + r)
+                start += mon
+            rates = growth_rates_monitor(np.array(rows), np.array(tmon),
+                                         [(int(ens.first[k]), int(ens.first[k + 1])) for k in range(R)])
+            growth = {"turns": growth_turns, "rings": R, "fit": "slope per turn of log <ampinv_cm>, HOR and VER",
+                      "unstable_rings": int(np.sum(np.nanmax(rates, axis=1) > 0)),
+                      "rate_per_turn": {"min": float(np.nanmin(rates)), "median": float(np.nanmedian(rates)),
+                                        "max": float(np.nanmax(rates))},
+                      "per_ring": [[round(float(v), 8) for v in r] for r in rates]}
         ens.close()
     sec = t1 - t0
     k_turn_ms = ms[0] / max(launches[0], 1)
@@ -219,7 +242,8 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(TRAFFIC_FILE_BATCHED)},
             "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
-                               "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)}}, last_stats
+                               "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)},
+            **({"growth_rates": growth} if growth else {})}, last_stats
 
 
 def fbii_conf(d, train=356, np_=10000, turns=2000):
@@ -310,6 +334,8 @@ def main():
                     help="work items per lane for the batched line (0: autotune)")
     ap.add_argument("--batched-mixed", action="store_true", help="with --no-production: the batched line in mixed precision")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision batched run")
+    ap.add_argument("--batched-growth-turns", type=int, default=0,
+                    help="after the timed turns, track this many more and report per-ring growth rates")
     ap.add_argument("--no-c1", action="store_true", help="skip the small-ring FP64 / mixed line")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production line (profiling of the other lines only)")
@@ -327,7 +353,7 @@ def main():
         out = {}
         if not a.no_batched:
             out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk,
-                                     a.batched_mixed)[0]
+                                     a.batched_mixed, a.batched_growth_turns)[0]
         if not a.no_fbii:
             out["fbii"] = fbii(a.fbii_steps, 2, cpu=False)
         print(json.dumps(out))
@@ -390,7 +416,8 @@ def main():
             out["c1"] = {"value": None, "error": str(e)[:300]}
     if not a.no_batched:
         try:
-            out["batched"], st64 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk)
+            out["batched"], st64 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk,
+                                           growth_turns=a.batched_growth_turns)
             if not a.no_mixed:
                 mix, st32 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed,
                                     out["batched"]["config"]["chunk_items"], mixed=True)
