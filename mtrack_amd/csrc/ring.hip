@@ -46,6 +46,9 @@
 #define CHUNK_ITEMS 8
 #endif
 #define NMOM 15
+// threads per bunch of the per-bunch resistive-wall sum: one tap in 1,024 per
+// thread keeps the (Nmlt+1) h taps of a bunch to ~20 loop trips
+#define RW_WG 1024
 #ifndef TURN_WAVES_PER_EU
 #define TURN_WAVES_PER_EU 4
 #endif
@@ -723,15 +726,29 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
   double *fnp = sm;
   double *dV = fnp + Nc;
   double *dH = dV + Nc;
+  double *Gs = dH + Nc;  // the ring's three Green's functions, [plane][Nc]
   __shared__ double mom[NMOM];
   __shared__ int icmin_s, icmax_s;
   const long long rev = ts->rev;
   const int w0 = bunch_wg0[b], w1 = bunch_wg0[b + 1];
   const double Np = (double)R.np[b];
-  if (tid < NMOM) {
+  {
+    // moment sums over the bunch's workgroups: 16 lanes per moment, each summing
+    // every 16th workgroup in order, then a fixed butterfly over the 16 lanes
+    const int q = tid >> 4, j = tid & 15;
     double s = 0.0;
-    for (int w = w0; w < w1; w++) s += partial[(size_t)w * NMOM + tid];
-    mom[tid] = s;
+    if (q < NMOM)
+      for (int w = w0 + j; w < w1; w += 16) s += partial[(size_t)w * NMOM + q];
+    for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (q < NMOM && j == 0) mom[q] = s;
+  }
+  if (phase != 0 && Nc > 0) {
+    const double *Gr = R.G + (size_t)ring * 3 * Nc;
+    for (int i = tid; i < 3 * Nc; i += WG) Gs[i] = Gr[i];
+  }
+  if (tid == 0) {
+    icmin_s = Nc - 1;
+    icmax_s = 0;
   }
   __syncthreads();
   const int slot = (int)(((rev % STATS_TURNS) + STATS_TURNS) % STATS_TURNS);
@@ -836,6 +853,12 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
       zl = nl;
     }
     fnp[i] = (double)c;
+    // icellmin / icellmax of src/transform_weak.c:633-651: the first and last
+    // occupied slice (Nc - 1 and 0 when there is none)
+    if (c > 0) {
+      atomicMin(&icmin_s, i);
+      atomicMax(&icmax_s, i);
+    }
     dV[i] = bad ? NAN : fx_to_double(zl, zh);
     dH[i] = bad ? NAN : fx_to_double(xl, xh);
     R.fnp_out[g] = fnp[i];
@@ -864,82 +887,63 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     H.tsx[4 * b + 2 * tid] = H.tsx[4 * b + 2 * tid + 1] = 0;
     H.tsz[4 * b + 2 * tid] = H.tsz[4 * b + 2 * tid + 1] = 0;
   }
-  if (tid == 0) {
-    H.bad[b] = 0;
-    // icellmin / icellmax searches of src/transform_weak.c:633-651
-    int mn = Nc - 1, mx = 0;
-    for (int i = 0; i < mn; i++)
-      if (fnp[i] > 0) {
-        mn = i;
-        break;
-      }
-    for (int i = Nc - 1; i > mx; i--)
-      if (fnp[i] > 0) {
-        mx = i;
-        break;
-      }
-    icmin_s = mn;
-    icmax_s = mx;
-  }
-  __syncthreads();
+  if (tid == 0) H.bad[b] = 0;
   const int icmin = icmin_s, icmax = icmax_s;
   // the next turn's kick uses the current after this turn's scan step (src/tracking.c:43-59)
   double Ib = R.bunch_Ib[b];
   if (phase == 2 && P.scan == 1 && (rev + 1) % P.NrevScan == 0) Ib += P.scan_step / (((double)P.Nb) * 1.0e3);
   const double factG = -P.T0 * Ib / (R.np[b] * P.E0 * 1.0e9);
   const double sgm = P.sigma_tau, sgm2 = sgm * sgm, dT2 = P.dT * P.dT;
-  const double *Gl = R.G + (size_t)ring * 3 * Nc, *Gv = Gl + Nc, *Gh = Gv + Nc;
-  for (int i = tid; i < Nc; i += WG) {
-    double GL1 = 0.0, GV = 0.0, GH = 0.0;
-    if (P.convL && i >= icmin) {
-      double Gn = 0.0;
-      if (P.mixed) {
-        F2 g = {0.0f, 0.0f};
-        for (int k = i; k >= icmin; k--) f2_add_prod(g, fnp[k], Gl[i - k]);
-        Gn = (double)g.hi + (double)g.lo;
-      } else {
-        for (int k = i; k >= icmin; k--) Gn += fnp[k] * Gl[i - k];
-      }
-      GL1 = 1 * Gn;
-      // resistive and inductive smoothing, src/transform_weak.c:670-683; reads
-      // outside [0, Ncell) count as zero
+  // one (plane, slice) convolution per task; tasks run from the last slice down
+  // so that the second sweep of the block gets the short ones
+  for (int t = tid; t < 3 * Nc; t += WG) {
+    const int pl = t % 3, i = Nc - 1 - t / 3;
+    const double *Gp = Gs + pl * Nc;
+    const double *src = pl == 0 ? fnp : (pl == 1 ? dV : dH);
+    double val = 0.0;
+    if (pl == 0) {
+      if (P.convL && i >= icmin) {
+        double Gn = 0.0;
+        if (P.mixed) {
+          F2 g = {0.0f, 0.0f};
+          for (int k = i; k >= icmin; k--) f2_add_prod(g, src[k], Gp[i - k]);
+          Gn = (double)g.hi + (double)g.lo;
+        } else {
+#pragma unroll 4
+          for (int k = i; k >= icmin; k--) Gn += src[k] * Gp[i - k];
+        }
+        double GL1 = 1 * Gn;
+        // resistive and inductive smoothing, src/transform_weak.c:670-683; reads
+        // outside [0, Ncell) count as zero
 #define F(q) (((q) >= 0 && (q) < Nc) ? fnp[q] : 0.0)
-      if (P.RsisL > 0 && i >= icmin + 1 && i <= icmax) {
-        double nt = (0.5 * F(i - 1) + F(i) + 0.5 * F(i + 1)) / 2.0;
-        GL1 += P.RsisL * nt / (sgm * P.dT);
-      }
-      if (P.aindL > 0 && i >= icmin + 1 && i <= icmax) {
-        double n0 = (0.25 * F(i - 3) + 0.5 * F(i - 2) + F(i - 1) + 0.5 * F(i) + 0.25 * F(i + 1)) / 2.5;
-        double n1 = (0.25 * F(i - 1) + 0.5 * F(i) + F(i + 1) + 0.5 * F(i + 2) + 0.25 * F(i + 3)) / 2.5;
-        GL1 += (n1 - n0) / (2.0 * sgm2 * dT2) * P.aindL;
-      }
+        if (P.RsisL > 0 && i >= icmin + 1 && i <= icmax) {
+          double nt = (0.5 * F(i - 1) + F(i) + 0.5 * F(i + 1)) / 2.0;
+          GL1 += P.RsisL * nt / (sgm * P.dT);
+        }
+        if (P.aindL > 0 && i >= icmin + 1 && i <= icmax) {
+          double n0 = (0.25 * F(i - 3) + 0.5 * F(i - 2) + F(i - 1) + 0.5 * F(i) + 0.25 * F(i + 1)) / 2.5;
+          double n1 = (0.25 * F(i - 1) + 0.5 * F(i) + F(i + 1) + 0.5 * F(i + 2) + 0.25 * F(i + 3)) / 2.5;
+          GL1 += (n1 - n0) / (2.0 * sgm2 * dT2) * P.aindL;
+        }
 #undef F
-    }
-    if (P.kickV) {
-      double Gn = 0.0;
-      if (P.mixed) {
-        F2 g = {0.0f, 0.0f};
-        for (int k = i; k >= 0; k--) f2_add_prod(g, dV[k], Gv[i - k]);
-        Gn = (double)g.hi + (double)g.lo;
-      } else {
-        for (int k = i; k >= 0; k--) Gn += dV[k] * Gv[i - k];
+        val = GL1;
       }
-      GV = 1 * Gn;
-    }
-    if (P.kickH && i >= 1) {
-      double Gn = 0.0;
-      if (P.mixed) {
-        F2 g = {0.0f, 0.0f};
-        for (int k = i; k >= 0; k--) f2_add_prod(g, dH[k], Gh[i - k]);
-        Gn = (double)g.hi + (double)g.lo;
-      } else {
-        for (int k = i; k >= 0; k--) Gn += dH[k] * Gh[i - k];
+      R.sfL[(size_t)b * Nc + i] = factG * val;
+    } else {
+      if (pl == 1 ? P.kickV : (P.kickH && i >= 1)) {
+        double Gn = 0.0;
+        if (P.mixed) {
+          F2 g = {0.0f, 0.0f};
+          for (int k = i; k >= 0; k--) f2_add_prod(g, src[k], Gp[i - k]);
+          Gn = (double)g.hi + (double)g.lo;
+        } else {
+#pragma unroll 4
+          for (int k = i; k >= 0; k--) Gn += src[k] * Gp[i - k];
+        }
+        val = 1 * Gn;
       }
-      GH = 1 * Gn;
+      (pl == 1 ? R.sfV : R.sfH)[(size_t)b * Nc + i] = -factG * val;
     }
-    R.sfL[(size_t)b * Nc + i] = factG * GL1;
-    R.sfV[(size_t)b * Nc + i] = -factG * GV;
-    R.sfH[(size_t)b * Nc + i] = -factG * GH;
   }
 }
 
@@ -963,21 +967,25 @@ __device__ void rw_sum(const DevParams &P, int b, const TurnState *ts, const dou
   const long long n = (long long)(P.Nmlt + 2) * h;
   const long long M = (long long)(P.Nmlt + 1) * h;
   // index of tap m in the cyclic history: (-rev h + kb + m) mod n, bucket (kb + m) mod h,
-  // advanced incrementally (stride WG) instead of by 64-bit divisions per tap
+  // advanced incrementally (stride RW_WG) instead of by 64-bit divisions per tap.
+  // Every load of a tap is unconditional (all indices are in range) so that the
+  // unrolled loop keeps several taps' loads in flight; an empty bucket adds +0.
   const long long m0 = 1 + tid;
   long long k0 = (-rev * h + kb + m0) % n;
   long long idx = k0 < 0 ? k0 + n : k0;
   int in_mod = (int)((kb + m0) % h);
-  const long long step_n = WG % n;
-  const int step_h = WG % h;
+  const long long step_n = RW_WG % n;
+  const int step_h = RW_WG % h;
   double ax = 0.0, az = 0.0;
-  for (long long m = m0; m <= M; m += WG) {
-    if (fill[in_mod]) {
-      const double I = Ib[in_mod];
-      const double st = stt[m];
-      ax += P.fNp * hist[2 * idx] * I / st;
-      az += P.fNp * hist[2 * idx + 1] * I / st;
-    }
+#pragma unroll 4
+  for (long long m = m0; m <= M; m += RW_WG) {
+    const int f = fill[in_mod];
+    const double I = Ib[in_mod];
+    const double st = stt[m];
+    const double hx = hist[2 * idx], hz = hist[2 * idx + 1];
+    const double tx = P.fNp * hx * I / st, tz = P.fNp * hz * I / st;
+    ax += f ? tx : 0.0;
+    az += f ? tz : 0.0;
     idx += step_n;
     if (idx >= n) idx -= n;
     in_mod += step_h;
@@ -992,7 +1000,7 @@ __device__ void rw_sum(const DevParams &P, int b, const TurnState *ts, const dou
   __syncthreads();
   if (tid == 0) {
     double sx = 0, sz = 0;
-    for (int w = 0; w < WG / 64; w++) {
+    for (int w = 0; w < RW_WG / 64; w++) {
       sx += red[w][0];
       sz += red[w][1];
     }
@@ -1048,12 +1056,12 @@ __device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_his
 
 // Per bunch: the resistive-wall sum; the last block of each ring then ends the
 // ring's turn (and, with fb_here, runs its RF-feedback step).
-__global__ __launch_bounds__(WG) void k_rw(const DevParams *__restrict__ Pr, const int *bunch_ring, const double *rwhist,
-                                           const double *sqrt_tt, double *Ibunch, const int *nfFill,
-                                           const int *bucket, double *pending, TurnState *tsr,
-                                           unsigned *ring_done, double *bunch_Ib, int fb_here)
+__global__ __launch_bounds__(RW_WG) void k_rw(const DevParams *__restrict__ Pr, const int *bunch_ring,
+                                              const double *rwhist, const double *sqrt_tt, double *Ibunch,
+                                              const int *nfFill, const int *bucket, double *pending,
+                                              TurnState *tsr, unsigned *ring_done, double *bunch_Ib, int fb_here)
 {
-  __shared__ double red[WG / 64][2];
+  __shared__ double red[RW_WG / 64][2];
   __shared__ int last;
   const int b = blockIdx.x, tid = threadIdx.x;
   const int ring = bunch_ring[b];
@@ -2230,8 +2238,8 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   Ro.R = R;
   r->use_graphs = !(flags & MBT_RING_NO_GRAPHS);
   r->lds_turn = turn_lds(Nc);
-  r->lds_red = (size_t)3 * NcA * 8 + 64;
-  if (r->lds_turn > 64 * 1024) {
+  r->lds_red = (size_t)6 * NcA * 8 + 64;
+  if (r->lds_turn > 64 * 1024 || r->lds_red > 64 * 1024) {
     mbt_set_error("Ncell %d too large for the LDS histogram", Nc);
     mbt_ring_destroy(r);
     return MBT_EINVAL;
@@ -2497,7 +2505,7 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                        !lr_launch);
   } else if (r->rw_total > 0)
-    hipLaunchKernelGGL(k_rw, dim3(r->G), dim3(WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
+    hipLaunchKernelGGL(k_rw, dim3(r->G), dim3(RW_WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
                        r->Ibunch, r->nfFill, r->bucket, r->pending, r->ts, r->ring_done, r->bunch_Ib, !lr_launch);
   else
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
