@@ -204,6 +204,12 @@ typedef struct mbt_ring mbt_ring;
                                    of FP32 chunks; a lone ring keeps its per-bunch FP64 RW sum,
                                    which is latency-bound; coordinates, phase advance and the
                                    LR / RF-feedback recurrence stay FP64 */
+#define MBT_RING_RW_OVERLAP 16  /* resistive-wall sum without the GEMM: always split it into the
+                                   taps of earlier turns' rows, summed on a second stream while
+                                   the turn's map runs, and the taps of the row the turn writes,
+                                   summed after the reduction.  Without the flag only rings of
+                                   at least 4,194,304 macroparticles split it; below that one
+                                   fused pass after the reduction is faster */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);

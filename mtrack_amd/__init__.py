@@ -20,6 +20,7 @@ MBT_RING_NO_INIT_DIST = 1
 MBT_RING_NO_GRAPHS = 2
 MBT_RING_NO_RW_GEMM = 4
 MBT_RING_MIXED = 8
+MBT_RING_RW_OVERLAP = 16
 MBT_WS_BUNCH_CURRENTS = 1
 MBT_WS_SERIAL_TRAIN = 2
 MBT_WS_MIXED = 4
@@ -92,9 +93,10 @@ def _check(rc):
         raise MbtError("mbtrack error %d: %s" % (rc, lib().mbt_last_error().decode()))
 
 
-def _flags(init_dist, graphs, rw_gemm=True, mixed=False):
+def _flags(init_dist, graphs, rw_gemm=True, mixed=False, rw_overlap=False):
     return ((0 if init_dist else MBT_RING_NO_INIT_DIST) | (0 if graphs else MBT_RING_NO_GRAPHS)
-            | (0 if rw_gemm else MBT_RING_NO_RW_GEMM) | (MBT_RING_MIXED if mixed else 0))
+            | (0 if rw_gemm else MBT_RING_NO_RW_GEMM) | (MBT_RING_MIXED if mixed else 0)
+            | (MBT_RING_RW_OVERLAP if rw_overlap else 0))
 
 
 def dptr(a):
@@ -150,14 +152,15 @@ class Config:
 class Ring:
     """All bunches of one ring resident on one GPU (mbt_ring_* C ABI)."""
 
-    def __init__(self, cfg, device=0, init_dist=True, graphs=True, mixed=False):
+    def __init__(self, cfg, device=0, init_dist=True, graphs=True, mixed=False, rw_overlap=False):
         L = lib()
         self.cfg = cfg
         self.Nb = cfg.geti("Nbunch")
         self.Np = cfg.geti("Np")
         self.Nc = cfg.geti("Ncell")
         p = ctypes.c_void_p()
-        _check(L.mbt_ring_create(cfg.ptr, device, _flags(init_dist, graphs, mixed=mixed), ctypes.byref(p)))
+        _check(L.mbt_ring_create(cfg.ptr, device, _flags(init_dist, graphs, mixed=mixed, rw_overlap=rw_overlap),
+                                 ctypes.byref(p)))
         self.ptr = p
 
     def close(self):
@@ -237,7 +240,7 @@ class Ensemble(Ring):
     Bunch ordinals, stats and ampinv rows run over the rings' bunches in ring
     order; rf rows hold one (phai0, Vrf0) pair per ring."""
 
-    def __init__(self, cfgs, device=0, init_dist=True, graphs=True, rw_gemm=True, mixed=False):
+    def __init__(self, cfgs, device=0, init_dist=True, graphs=True, rw_gemm=True, mixed=False, rw_overlap=False):
         L = lib()
         self.cfgs = list(cfgs)
         self.R = len(self.cfgs)
@@ -249,7 +252,8 @@ class Ensemble(Ring):
         self.first = np.concatenate([[0], np.cumsum(self.nb)])
         arr = (ctypes.c_void_p * self.R)(*[c.ptr.value for c in self.cfgs])
         p = ctypes.c_void_p()
-        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs, rw_gemm, mixed), ctypes.byref(p)))
+        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs, rw_gemm, mixed, rw_overlap),
+                                     ctypes.byref(p)))
         self.ptr = p
 
     def size(self):

@@ -5,7 +5,8 @@
 // rings (one ring is an ensemble of one): per-ring constants sit in a DevParams
 // table indexed through bunch -> ring, so every launch covers all rings and
 // rings of uneven bunch and macroparticle counts share each kernel.  One turn
-// is a few kernels on one stream (replayed as a HIP graph per block of turns):
+// is a few kernels on one stream, one of them forked to a second stream
+// (replayed as a HIP graph per block of turns):
 //
 //   k_turn    one pass over every macroparticle: pending RW kick, selffield
 //             kick (src/transform_weak.c:697-731), one-turn map
@@ -14,11 +15,14 @@
 //   k_reduce  per bunch: statistics (src/statistics.c:91-195), RW dipole history,
 //             LR histograms (src/transform_weak.c:839-865) and the selffield
 //             convolutions for the next turn (src/transform_weak.c:633-731);
-//   k_rw      per bunch: resistive-wall long-range sum over (Nmlt+1) h past
+//   k_rw_old, k_rw_fresh, k_end
+//             per bunch: resistive-wall long-range sum over (Nmlt+1) h past
 //             buckets (src/transform_weak.c:780-833), applied as a pending kick
-//             at the start of the next pass, and the turn end; for ensembles of
-//             rings sharing one tap table the sum is one MFMA GEMM instead
-//             (k_rwg_y, k_rwg_tile, k_rwg_sum, then k_end);
+//             at the start of the next pass: the taps of earlier turns' rows on
+//             the second stream during k_turn, this turn's row after k_reduce;
+//             then the turn end.  For ensembles of rings sharing one tap table
+//             the sum is one MFMA GEMM instead (k_rwg_y, k_rwg_tile, k_rwg_sum,
+//             then k_end);
 //   k_lr_par  with long-range resonators: the phasor recurrence over all buckets
 //             and the RF-feedback samples (construct_wake_phasor
 //             src/transform_weak.c:961-1048), linear, so one block per resonator
@@ -46,9 +50,16 @@
 #define CHUNK_ITEMS 8
 #endif
 #define NMOM 15
-// threads per bunch of the per-bunch resistive-wall sum: one tap in 1,024 per
-// thread keeps the (Nmlt+1) h taps of a bunch to ~20 loop trips
+// threads per bunch of the one-pass resistive-wall sum of small rings
 #define RW_WG 1024
+// macroparticles from which the RW sum of earlier turns' rows runs beside the
+// map on a second stream (below, the fork and the extra launch cost more than
+// the overlap saves)
+#define RW_SPLIT_MIN_MP (1LL << 22)
+// consecutive taps per k_rw_old tile, and its LDS budget (the slots of the
+// tile's bunches and taps, shared with k_turn's blocks on the same CU)
+#define RW_TILE_TAPS 256
+#define RW_OLD_LDS_MAX (48 * 1024)
 #ifndef TURN_WAVES_PER_EU
 #define TURN_WAVES_PER_EU 4
 #endif
@@ -947,38 +958,42 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
   }
 }
 
-// ------------------------------------------------------------------- k_rw
+// ------------------------------------------------------- k_rw_old, k_rw_fresh
 
 // Resistive-wall long-range kick of bunch b (src/transform_weak.c:780-833): the
-// sum over (Nmlt+1) h past buckets of the ring's dipole history, stored as the
-// pending kick the next pass applies.
-__device__ void rw_sum(const DevParams &P, int b, const TurnState *ts, const double *rwhist, const double *sqrt_tt,
-                       const double *Ibunch, const int *nfFill, const int *bucket, double *pending,
-                       double (*red)[2])
+// sum over the taps m = 1 .. (Nmlt+1) h of the ring's dipole history, tap m
+// reading the bucket m places after b's.  Only the taps m < h - kb read the row
+// this turn's k_reduce writes; the others read rows of earlier turns.  So the sum
+// is split: k_rw_old adds the earlier rows' taps on a second stream while the
+// turn's map runs, k_rw_fresh adds the fresh row's taps after k_reduce and stores
+// the pending kick the next pass applies.
+//
+// rw_taps: NT lanes sum taps [mlo, mhi]; the block total ends in (sx, sz) of lane 0.
+template <int NT, int UNR>
+__device__ void rw_taps(const DevParams &P, int kb, long long rev, const double *rwhist, const double *sqrt_tt,
+                        const double *Ibunch, const int *nfFill, long long mlo, long long mhi, double (*red)[2],
+                        double &sx, double &sz)
 {
   const int tid = threadIdx.x;
-  const int kb = bucket[b];
-  const long long rev = ts->rev;
   const int h = P.h;
   const double *hist = rwhist + P.rw_off;
   const double *stt = sqrt_tt + P.st_off;
   const double *Ib = Ibunch + P.bucket_off;
   const int *fill = nfFill + P.bucket_off;
   const long long n = (long long)(P.Nmlt + 2) * h;
-  const long long M = (long long)(P.Nmlt + 1) * h;
   // index of tap m in the cyclic history: (-rev h + kb + m) mod n, bucket (kb + m) mod h,
-  // advanced incrementally (stride RW_WG) instead of by 64-bit divisions per tap.
+  // advanced incrementally (stride NT) instead of by 64-bit divisions per tap.
   // Every load of a tap is unconditional (all indices are in range) so that the
   // unrolled loop keeps several taps' loads in flight; an empty bucket adds +0.
-  const long long m0 = 1 + tid;
+  const long long m0 = mlo + tid;
   long long k0 = (-rev * h + kb + m0) % n;
   long long idx = k0 < 0 ? k0 + n : k0;
   int in_mod = (int)((kb + m0) % h);
-  const long long step_n = RW_WG % n;
-  const int step_h = RW_WG % h;
+  const long long step_n = NT % n;
+  const int step_h = NT % h;
   double ax = 0.0, az = 0.0;
-#pragma unroll 4
-  for (long long m = m0; m <= M; m += RW_WG) {
+#pragma unroll UNR
+  for (long long m = m0; m <= mhi; m += NT) {
     const int f = fill[in_mod];
     const double I = Ib[in_mod];
     const double st = stt[m];
@@ -998,17 +1013,13 @@ __device__ void rw_sum(const DevParams &P, int b, const TurnState *ts, const dou
     red[tid >> 6][1] = az;
   }
   __syncthreads();
-  if (tid == 0) {
-    double sx = 0, sz = 0;
-    for (int w = 0; w < RW_WG / 64; w++) {
+  sx = 0.0;
+  sz = 0.0;
+  if (tid == 0)
+    for (int w = 0; w < NT / 64; w++) {
       sx += red[w][0];
       sz += red[w][1];
     }
-    int trig = ts->m < 50 ? 1 : 0;
-    pending[2 * b] = P.track_h ? sx * P.rwconst[0][trig] : 0.0;
-    pending[2 * b + 1] = P.track_v ? sz * P.rwconst[1][trig] : 0.0;
-  }
-  __syncthreads();
 }
 
 // Turn end of one ring: RW trigger counter, current scan (src/tracking.c:43-59),
@@ -1054,8 +1065,117 @@ __device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_his
   if (init && P.fb_switch_off) ts->has_fb = 0;
 }
 
-// Per bunch: the resistive-wall sum; the last block of each ring then ends the
-// ring's turn (and, with fb_here, runs its RF-feedback step).
+// The resistive-wall taps of the rows of earlier turns (second stream, concurrent
+// with the turn's map; rev and the bucket currents change only at the turn's
+// end).  One block per tile: up to WG bunches of one ring (one lane each) times
+// RW_TILE_TAPS consecutive taps.  Tap m of the bunch in bucket kb reads history
+// slot u = kb + m (slot (-rev h + u) mod n of the cyclic buffer, bucket u mod h),
+// so the tile's bunches share the span of slots [kb_lo + m0, kb_hi + m1], staged
+// once in LDS; the loads per turn are the history once per tile instead of once
+// per bunch.  Each term is the reference's fNp * dip * I / sqrt(t) (the division
+// is kept); one plane per sweep keeps it to 32 VGPRs, so that its waves fit
+// beside k_turn's (4 waves of <= 120 VGPRs per SIMD).
+__global__ __launch_bounds__(WG) void k_rw_old(const DevParams *__restrict__ Pr, const int4 *tiles,
+                                               const double *rwhist, const double *sqrt_tt, const double *Ibunch,
+                                               const int *nfFill, const int *bucket,
+                                               const TurnState *__restrict__ tsr, double *rw_part, int G)
+{
+  extern __shared__ double sm[];
+  const int4 T = tiles[blockIdx.x]; // ring, first bunch, bunches, tap tile
+  const DevParams &P = Pr[T.x];
+  const int tid = threadIdx.x;
+  const int h = P.h, n = (P.Nmlt + 2) * h, M = (P.Nmlt + 1) * h;
+  const long long rev = tsr[T.x].rev;
+  const int m0 = 1 + T.w * RW_TILE_TAPS;
+  const int m1 = min(m0 + RW_TILE_TAPS - 1, M);
+  const int kb_lo = bucket[T.y], kb_hi = bucket[T.y + T.z - 1];
+  const int u0 = kb_lo + m0, span = kb_hi - kb_lo + m1 - m0 + 1;
+  double *lx = sm, *lz = lx + span, *lI = lz + span, *lst = lI + span;
+  int *lf = (int *)(lst + RW_TILE_TAPS);
+  const double *hist = rwhist + P.rw_off;
+  const double *Ib = Ibunch + P.bucket_off;
+  const int *fill = nfFill + P.bucket_off;
+  const long long k0 = (-rev * h + u0) % n;
+  const int base = (int)(k0 < 0 ? k0 + n : k0);
+  for (int e = tid; e < span; e += WG) {
+    int idx = base + e;
+    if (idx >= n) idx -= n;
+    const int bk = (u0 + e) % h;
+    lx[e] = hist[2 * idx];
+    lz[e] = hist[2 * idx + 1];
+    lI[e] = Ib[bk];
+    lf[e] = fill[bk];
+  }
+  for (int i = tid; i <= m1 - m0; i += WG) lst[i] = sqrt_tt[P.st_off + m0 + i];
+  __syncthreads();
+  if (tid >= T.z) return;
+  const int b = T.y + tid, kb = bucket[b];
+  const int mlo = max(m0, h - kb); // earlier rows only: m >= h - kb
+  const double fNp = P.fNp;
+#pragma unroll 1
+  for (int pl = 0; pl < 2; pl++) {
+    const double *lv = pl ? lz : lx;
+    double a = 0.0;
+#pragma unroll 1
+    for (int m = mlo; m <= m1; m++) {
+      const int e = kb + m - u0;
+      const double t = fNp * lv[e] * lI[e] / lst[m - m0];
+      a += lf[e] ? t : 0.0;
+    }
+    rw_part[((size_t)T.w * G + b) * 2 + pl] = a;
+  }
+}
+
+static size_t rw_old_lds(int span) { return (size_t)span * (3 * 8 + 4) + RW_TILE_TAPS * 8; }
+
+// Per bunch, after k_reduce: the taps of this turn's row (m < h - kb) plus the
+// earlier rows' tile partials (in tile order), times the RW constant of the
+// trigger state.
+__global__ __launch_bounds__(WG) void k_rw_fresh(const DevParams *__restrict__ Pr, const int *bunch_ring,
+                                                 const double *rwhist, const double *sqrt_tt, const double *Ibunch,
+                                                 const int *nfFill, const int *bucket,
+                                                 const TurnState *__restrict__ tsr, const double *rw_part, int G,
+                                                 double *pending)
+{
+  __shared__ double red[WG / 64][2];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int ring = bunch_ring[b];
+  const DevParams &P = Pr[ring];
+  if (!P.rw) return;
+  const int kb = bucket[b];
+  const TurnState *ts = tsr + ring;
+  double fx, fz;
+  rw_taps<WG, 4>(P, kb, ts->rev, rwhist, sqrt_tt, Ibunch, nfFill, 1, (long long)(P.h - kb - 1), red, fx, fz);
+  __syncthreads();
+  // earlier rows: one tile partial per lane (fixed order)
+  const int ntile = ((P.Nmlt + 1) * P.h + RW_TILE_TAPS - 1) / RW_TILE_TAPS;
+  double ox = 0.0, oz = 0.0;
+  for (int c = tid; c < ntile; c += WG) {
+    ox += rw_part[((size_t)c * G + b) * 2];
+    oz += rw_part[((size_t)c * G + b) * 2 + 1];
+  }
+  ox = wave_sum(ox);
+  oz = wave_sum(oz);
+  if ((tid & 63) == 0) {
+    red[tid >> 6][0] = ox;
+    red[tid >> 6][1] = oz;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sx = 0.0, sz = 0.0;
+    for (int w = 0; w < WG / 64; w++) {
+      sx += red[w][0];
+      sz += red[w][1];
+    }
+    const int trig = ts->m < 50 ? 1 : 0;
+    pending[2 * b] = P.track_h ? (sx + fx) * P.rwconst[0][trig] : 0.0;
+    pending[2 * b + 1] = P.track_v ? (sz + fz) * P.rwconst[1][trig] : 0.0;
+  }
+}
+
+// Small rings (the map too short to hide k_rw_old): every tap in one pass, and
+// the ring's last block to finish ends the turn (every block of the ring has read
+// the turn state and the bucket currents before its increment).
 __global__ __launch_bounds__(RW_WG) void k_rw(const DevParams *__restrict__ Pr, const int *bunch_ring,
                                               const double *rwhist, const double *sqrt_tt, double *Ibunch,
                                               const int *nfFill, const int *bucket, double *pending,
@@ -1067,9 +1187,17 @@ __global__ __launch_bounds__(RW_WG) void k_rw(const DevParams *__restrict__ Pr, 
   const int ring = bunch_ring[b];
   const DevParams &P = Pr[ring];
   TurnState *ts = tsr + ring;
-  if (P.rw) rw_sum(P, b, ts, rwhist, sqrt_tt, Ibunch, nfFill, bucket, pending, red);
-  // the ring's last block to finish ends the turn: every block of the ring has
-  // read the turn state and the bucket currents before its increment
+  if (P.rw) {
+    const int kb = bucket[b];
+    double sx, sz;
+    rw_taps<RW_WG, 4>(P, kb, ts->rev, rwhist, sqrt_tt, Ibunch, nfFill, 1, (long long)(P.Nmlt + 1) * P.h, red, sx,
+                      sz);
+    if (tid == 0) {
+      const int trig = ts->m < 50 ? 1 : 0;
+      pending[2 * b] = P.track_h ? sx * P.rwconst[0][trig] : 0.0;
+      pending[2 * b + 1] = P.track_v ? sz * P.rwconst[1][trig] : 0.0;
+    }
+  }
   if (tid == 0) {
     __threadfence();
     last = atomicAdd(&ring_done[ring], 1u) == (unsigned)(P.Nb - 1);
@@ -1579,7 +1707,14 @@ struct mbt_ring {
   double *fnp_out, *dipV_out, *dipH_out;
   int *trash_tot, *trash_pend;
   TurnState *ts;
-  unsigned *ring_done; // [R] blocks of the ring done with the turn's RW pass
+  int4 *rw_tiles;      // k_rw_old tiles: ring, first bunch, bunches, tap tile
+  double *rw_part;     // [tap tiles][G][2] partial sums of earlier turns' rows
+  int rw_ntile;
+  size_t rw_lds;
+  unsigned *ring_done; // [R] blocks of the ring done with the turn's fused RW pass
+  int rw_split;        // RW sum split into k_rw_old (side stream) and k_rw_fresh
+  hipStream_t side;    // k_rw_old, concurrent with the turn's map
+  hipEvent_t ev_fork, ev_join;
   GraphEntry graphs[NGRAPH];
   int graph_next, use_graphs;
   LRConst *lc;
@@ -1744,7 +1879,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
                   r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
-                  r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
+                  r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->rw_tiles, r->rw_part, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
                   r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part, r->rwg_Y};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
     if (ptrs[i]) (void)hipFree(ptrs[i]);
@@ -1754,6 +1889,9 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
   }
   for (int i = 0; i < NGRAPH; i++)
     if (r->graphs[i].exec) (void)hipGraphExecDestroy(r->graphs[i].exec);
+  if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
+  if (r->ev_join) (void)hipEventDestroy(r->ev_join);
+  if (r->side) (void)hipStreamDestroy(r->side);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   if (r->rh)
     for (int k = 0; k < r->R; k++) {
@@ -1934,6 +2072,53 @@ static int setup_rw_gemm(mbt_ring *r)
   return MBT_OK;
 }
 
+// Tiles of k_rw_old: per RW ring, groups of up to WG consecutive bunches times
+// RW_TILE_TAPS consecutive taps.  Leaves rw_split off (the one-pass sum) when a
+// tile's slot span does not fit the LDS budget.
+static int setup_rw_split(mbt_ring *r)
+{
+  int ntile = 0, cmax = 0;
+  size_t lds = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int4 *tl = pass ? (int4 *)malloc(sizeof(int4) * (ntile > 0 ? ntile : 1)) : NULL;
+    int t = 0;
+    for (int k = 0; k < r->R; k++) {
+      const mbt_config *c = &r->rh[k].cfg;
+      if (!r->Ph[k].rw) continue;
+      const int h = (int)c->Nharm, M = (c->Nmlt + 1) * h;
+      const int C = (M + RW_TILE_TAPS - 1) / RW_TILE_TAPS;
+      if (C > cmax) cmax = C;
+      for (int g0 = 0; g0 < c->Nbunch; g0 += WG) {
+        const int nb = c->Nbunch - g0 < WG ? c->Nbunch - g0 : WG;
+        const int span_b = c->bucket[g0 + nb - 1] - c->bucket[g0];
+        for (int ct = 0; ct < C; ct++) {
+          const int m0 = 1 + ct * RW_TILE_TAPS, m1 = m0 + RW_TILE_TAPS - 1 < M ? m0 + RW_TILE_TAPS - 1 : M;
+          const size_t l = rw_old_lds(span_b + m1 - m0 + 1);
+          if (l > lds) lds = l;
+          if (pass) tl[t] = make_int4(k, r->Ph[k].bunch_off + g0, nb, ct);
+          t++;
+        }
+      }
+    }
+    if (!pass) {
+      ntile = t;
+      if (ntile == 0 || lds > RW_OLD_LDS_MAX) {
+        r->rw_split = 0;
+        return MBT_OK;
+      }
+      continue;
+    }
+    int rc = dalloc(&r->rw_tiles, (size_t)ntile);
+    if (!rc) rc = dalloc(&r->rw_part, (size_t)cmax * r->G * 2);
+    if (!rc && hipMemcpy(r->rw_tiles, tl, sizeof(int4) * ntile, hipMemcpyHostToDevice) != hipSuccess) rc = MBT_EDEVICE;
+    free(tl);
+    if (rc) return rc;
+  }
+  r->rw_ntile = ntile;
+  r->rw_lds = lds;
+  return MBT_OK;
+}
+
 static int create_rings(const mbt_config *const *cfgs, int R, int device, int flags, mbt_ring **out)
 {
   *out = NULL;
@@ -1965,6 +2150,9 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   // a blocking stream: ordered after work on the null stream (allocation-time
   // memsets) like every copy the library makes
   CK(hipStreamCreate(&r->stream));
+  CK(hipStreamCreate(&r->side));
+  CK(hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming));
   const int Nc = cfgs[0]->Ncell, NcA = Nc > 0 ? Nc : 1;
   r->Nc = Nc;
   r->nrev_mon = (int)cfgs[0]->NrevMon;
@@ -1996,6 +2184,13 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     Ntot += (long long)c->Np * c->Nbunch;
     Nalloc += padded_np(c->Np) * c->Nbunch;
     if (c->EnableRW_long) {
+      if ((long long)(c->Nmlt + 2) * c->Nharm >= (1LL << 30)) {
+        // the RW sum kernels index the history with 32-bit integers
+        mbt_set_error("resistive-wall history of %lld slots too long", (long long)(c->Nmlt + 2) * c->Nharm);
+        free(Gf);
+        mbt_ring_destroy(r);
+        return MBT_EINVAL;
+      }
       rw_total += 2LL * (c->Nmlt + 2) * c->Nharm;
       st_total += (long long)(c->Nmlt + 1) * c->Nharm + 1;
     }
@@ -2271,6 +2466,8 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   }
   int rcd = set_decomposition(r, CHUNK_ITEMS);
   if (!rcd && !(flags & MBT_RING_NO_RW_GEMM)) rcd = setup_rw_gemm(r);
+  r->rw_split = !r->rwg_on && r->rw_total > 0 && (r->Ntot >= RW_SPLIT_MIN_MP || (flags & MBT_RING_RW_OVERLAP));
+  if (!rcd && r->rw_split) rcd = setup_rw_split(r);
   if (rcd) {
     mbt_ring_destroy(r);
     return rcd;
@@ -2481,7 +2678,16 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
 {
   int rc;
   const int lr_launch = r->sf_any && r->nres > 0;
+  const int rw_split = r->rw_split;
   if (E) CK(hipEventRecord(E[0], r->stream));
+  if (rw_split) {
+    CK(hipEventRecord(r->ev_fork, r->stream));
+    CK(hipStreamWaitEvent(r->side, r->ev_fork, 0));
+    hipLaunchKernelGGL(k_rw_old, dim3(r->rw_ntile), dim3(WG), r->rw_lds, r->side, r->P, r->rw_tiles, r->rwhist,
+                       r->sqrt_tt, r->Ibunch, r->nfFill, r->bucket, r->ts, r->rw_part, r->G);
+    CK(hipGetLastError());
+    CK(hipEventRecord(r->ev_join, r->side));
+  }
   if ((rc = launch_turn_pass(r, MODE_TURN))) return rc;
   if (E) CK(hipEventRecord(E[1], r->stream));
   if ((rc = launch_reduce(r, 2))) return rc;
@@ -2501,6 +2707,13 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
     const long long tot = (long long)r->rwg.h * r->rwg.N;
     hipLaunchKernelGGL(k_rwg_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
                        r->pending);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
+                       !lr_launch);
+  } else if (rw_split) {
+    CK(hipStreamWaitEvent(r->stream, r->ev_join, 0));
+    hipLaunchKernelGGL(k_rw_fresh, dim3(r->G), dim3(WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
+                       r->Ibunch, r->nfFill, r->bucket, r->ts, r->rw_part, r->G, r->pending);
     CK(hipGetLastError());
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                        !lr_launch);

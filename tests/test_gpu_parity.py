@@ -156,3 +156,38 @@ def test_qe_statistics_match_reference_stream(tmp_path):
         same.max(), np.round(eq, 4), np.round(curve, 4)), flush=True)
     assert same.max() <= 1e-9
     assert np.all(eq <= EQ_TOL) and np.all(curve <= CURVE_TOL)
+
+
+@pytest.mark.parametrize("name", ["wakes", "fromfile", "scan", "edge_single"])
+def test_rw_overlap_matches_oracle(tmp_path, name):
+    """The resistive-wall sum split over two streams (the taps of earlier turns'
+    rows beside the map, the fresh row's taps after the reduction; what rings of
+    >= 4,194,304 macroparticles run) gives the oracle's turns, and replaying the
+    turns as graphs (a fork and a join per turn) equals launching them eagerly
+    bit for bit."""
+    path = write_variant(name, str(tmp_path))
+    print("case rw overlap", name, flush=True)
+    cfg = m.Config(path, str(tmp_path) + "/")
+    n = cfg.geti("NrevTot")
+    orc = o.OracleRing(cfg, o.RNG_PHILOX)
+    orc.prepare()
+    ost, oam, _ = orc.track(n)
+    runs = []
+    for graphs in (True, False):
+        g = m.Ring(cfg, graphs=graphs, rw_overlap=True)
+        g.prepare()
+        gst, gam, grf = g.track(n)
+        hc, tr = g.histograms()
+        runs.append((gst, gam, grf, hc, tr, g.get_bunch(g.Nb - 1)))
+        g.close()
+    gst, gam = runs[0][0], runs[0][1]
+    e = stat_errors(gst, ost)
+    print("%s: rw overlap turn err max %.3g" % (name, e.max()), flush=True)
+    assert e.max() <= RTOL
+    mon = (np.arange(len(ost)) + 1) % cfg.geti("NrevMon") == 0
+    ea = np.abs(gam[mon] - oam[mon]) / np.maximum(np.abs(oam[mon]), 1e-300)
+    assert ea.max() <= RTOL
+    hc_o, tr_o = orc.histograms()
+    assert np.array_equal(runs[0][3], hc_o) and np.array_equal(runs[0][4], tr_o)
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
