@@ -165,6 +165,15 @@ def c1(turns=2000):
     return out
 
 
+def placement(achieved, ceiling):
+    """k_turn's rate against a plain in-place update of the same six coordinate
+    arrays (mbt_ring_stream_ceiling): the ceiling the arrays' placement in HBM
+    sets for this traffic (it differs between allocations, 5.2-5.4 or 6.1-6.3
+    TB/s on the devices measured)."""
+    return {"GB/s": ceiling, "k_turn_frac": achieved / ceiling,
+            "what": "plain in-place update of the same six coordinate arrays, bit-preserving, 96 B per slot"}
+
+
 def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
     """The batched ensemble: nrings independent rings (bunch and macroparticle
     counts log-uniform, currents and chromaticities uniform, RF beam-loading
@@ -201,6 +210,8 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
         t1 = time.perf_counter()
         ms, launches = ens.kernel_times()
         bytes_turn = ens.bytes_per_turn()
+        ens.set_timing(False)
+        ceiling = ens.stream_ceiling(5)
         growth = None
         if growth_turns > 0:
             # per-ring growth rates of the centroid action over further turns,
@@ -243,6 +254,7 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
                          "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(TRAFFIC_FILE_BATCHED)},
             "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
                                "ms_per_step": 1e3 * (g1 - g0) / steps, "value": N * steps / (g1 - g0)},
+            "placement_ceiling": placement(achieved, ceiling),
             **({"growth_rates": growth} if growth else {})}, last_stats
 
 
@@ -379,6 +391,8 @@ def main():
         t1 = time.perf_counter()
         ms, launches = ring.kernel_times()
         bytes_turn = ring.bytes_per_turn()
+        ring.set_timing(False)
+        ceiling = ring.stream_ceiling(10)
         ring.close()
     sec = t1 - t0
     value = nb * npb * a.steps / sec
@@ -408,6 +422,7 @@ def main():
         "uninstrumented": {"note": "same turns replayed as HIP graphs, no timing events",
                            "graphs": not a.no_graphs, "ms_per_step": 1e3 * (g1 - g0) / a.steps,
                            "value": nb * npb * a.steps / (g1 - g0)},
+        "placement_ceiling": placement(achieved, ceiling),
     }
     if not a.no_c1:
         try:

@@ -262,6 +262,12 @@ int mbt_ring_get_chunk(const mbt_ring *r);
  * the coordinates written back unchanged, so the ring's state is untouched --
  * keeps the fastest and returns it in *best_items.  Call after mbt_ring_prepare. */
 int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
+/* No counterpart in the reference (a measurement aid): the rate in GB/s of a
+ * plain in-place update of the ring's own six coordinate arrays (96 bytes per
+ * slot, every bit written back unchanged, so the ring's state is untouched),
+ * averaged over reps passes after one untimed pass -- the ceiling the arrays'
+ * placement in device memory sets for the per-macroparticle pass's traffic. */
+int mbt_ring_stream_ceiling(mbt_ring *r, int reps, double *gbps);
 
 /* ---- ensembles of independent rings on one GPU ------------------------------ */
 /* No counterpart in the reference, which runs one ring per MPI job: nrings

@@ -71,6 +71,7 @@ def lib():
         L.mbt_ring_set_chunk.argtypes = [vp, ip]
         L.mbt_ring_get_chunk.argtypes = [vp]
         L.mbt_ring_autotune.argtypes = [vp, ctypes.POINTER(ip), dp]
+        L.mbt_ring_stream_ceiling.argtypes = [vp, ctypes.c_int, dp]
         L.mbt_ring_rf_state.argtypes = [vp, ip, dp]
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
@@ -232,6 +233,13 @@ class Ring:
         ms = np.zeros(4)
         _check(lib().mbt_ring_autotune(self.ptr, ctypes.byref(best), dptr(ms)))
         return best.value, dict(zip((4, 8, 16, 32), ms.tolist()))
+
+    def stream_ceiling(self, reps=10):
+        """GB/s of a plain in-place update of the ring's own coordinate arrays
+        (bit-preserving): the ceiling their placement sets for the turn pass."""
+        out = ctypes.c_double()
+        _check(lib().mbt_ring_stream_ceiling(self.ptr, reps, ctypes.byref(out)))
+        return out.value
 
 
 class Ensemble(Ring):

@@ -191,3 +191,25 @@ def test_rw_overlap_matches_oracle(tmp_path, name):
     assert np.array_equal(runs[0][3], hc_o) and np.array_equal(runs[0][4], tr_o)
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
+
+
+def test_stream_ceiling_leaves_state(tmp_path):
+    """The placement-ceiling probe rewrites every coordinate bit for bit: the
+    ring's bunches and its further turns equal those of a ring never probed."""
+    path = write_variant("wakes", str(tmp_path))
+    print("case stream ceiling", flush=True)
+    out = []
+    for probe in (False, True):
+        cfg = m.Config(path, str(tmp_path) + "/")
+        g = m.Ring(cfg)
+        g.prepare()
+        g.track(5)
+        if probe:
+            gbps = g.stream_ceiling(3)
+            print("ceiling %.1f GB/s" % gbps, flush=True)
+            assert gbps > 0
+        st, am, _ = g.track(10)
+        out.append((st, am, g.get_bunch(0), g.get_bunch(g.Nb - 1)))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
