@@ -205,8 +205,8 @@ typedef struct mbt_ring mbt_ring;
                                    which is latency-bound; coordinates, phase advance and the
                                    LR / RF-feedback recurrence stay FP64 */
 #define MBT_RING_RW_OVERLAP 16  /* resistive-wall sum without the GEMM: always split it into the
-                                   taps of earlier turns' rows, summed on a second stream while
-                                   the turn's map runs, and the taps of the row the turn writes,
+                                   taps of earlier turns' rows, summed by the leading blocks of
+                                   the turn's map, and the taps of the row the turn writes,
                                    summed after the reduction.  Without the flag only rings of
                                    at least 4,194,304 macroparticles split it; below that one
                                    fused pass after the reduction is faster */

@@ -5,8 +5,7 @@
 // rings (one ring is an ensemble of one): per-ring constants sit in a DevParams
 // table indexed through bunch -> ring, so every launch covers all rings and
 // rings of uneven bunch and macroparticle counts share each kernel.  One turn
-// is a few kernels on one stream, one of them forked to a second stream
-// (replayed as a HIP graph per block of turns):
+// is a few kernels on one stream (replayed as a HIP graph per block of turns):
 //
 //   k_turn    one pass over every macroparticle: pending RW kick, selffield
 //             kick (src/transform_weak.c:697-731), one-turn map
@@ -15,14 +14,14 @@
 //   k_reduce  per bunch: statistics (src/statistics.c:91-195), RW dipole history,
 //             LR histograms (src/transform_weak.c:839-865) and the selffield
 //             convolutions for the next turn (src/transform_weak.c:633-731);
-//   k_rw_old, k_rw_fresh, k_end
+//   k_rw_fresh, k_end (large rings) or k_rw (small rings)
 //             per bunch: resistive-wall long-range sum over (Nmlt+1) h past
 //             buckets (src/transform_weak.c:780-833), applied as a pending kick
-//             at the start of the next pass: the taps of earlier turns' rows on
-//             the second stream during k_turn, this turn's row after k_reduce;
-//             then the turn end.  For ensembles of rings sharing one tap table
-//             the sum is one MFMA GEMM instead (k_rwg_y, k_rwg_tile, k_rwg_sum,
-//             then k_end);
+//             at the start of the next pass; for large rings the taps of earlier
+//             turns' rows are summed by k_turn's leading blocks during the map
+//             and k_rw_fresh adds this turn's row after k_reduce; then the turn
+//             end.  For ensembles of rings sharing one tap table the sum is one
+//             MFMA GEMM instead (k_rwg_y, k_rwg_tile, k_rwg_sum, then k_end);
 //   k_lr_par  with long-range resonators: the phasor recurrence over all buckets
 //             and the RF-feedback samples (construct_wake_phasor
 //             src/transform_weak.c:961-1048), linear, so one block per resonator
@@ -52,15 +51,15 @@
 #define NMOM 15
 // threads per bunch of the one-pass resistive-wall sum of small rings
 #define RW_WG 1024
-// macroparticles from which the RW sum of earlier turns' rows runs beside the
-// map on a second stream (below, the fork and the extra launch cost more than
-// the overlap saves)
+// macroparticles from which the RW taps of earlier turns' rows run in k_turn's
+// leading blocks (below, the map is too short to hide them and the two-kernel
+// tail costs more than the one-pass sum)
 #define RW_SPLIT_MIN_MP (1LL << 22)
 // coordinate footprints from which the set-up reserves and releases the whole
 // footprint before allocating the six arrays
 #define COORD_RESERVE_MIN (1ULL << 30)
-// consecutive taps per k_rw_old tile, and its LDS budget (the slots of the
-// tile's bunches and taps, shared with k_turn's blocks on the same CU)
+// consecutive taps per RW tile of k_turn's leading blocks, and its LDS budget
+// (the slots of the tile's bunches and taps; k_turn's dynamic LDS grows to it)
 #define RW_TILE_TAPS 256
 #define RW_OLD_LDS_MAX (48 * 1024)
 #ifndef TURN_WAVES_PER_EU
@@ -418,6 +417,75 @@ __device__ __forceinline__ int xcd_remap(int bid, int n)
   return x * q + (x < r ? x : r) + i;
 }
 
+// Arguments of the resistive-wall tiles k_turn's leading blocks run (nlead = 0:
+// none): the taps of the history rows written in earlier turns.
+struct RwOld {
+  const int4 *tiles; // ring, first bunch, bunches, tap tile
+  int ntile, nlead;  // tiles; leading blocks (ntile rounded up to whole rounds of the 8 XCDs)
+  const double *rwhist, *sqrt_tt, *Ibunch;
+  const int *nfFill;
+  double *rw_part; // [tap tiles][G][2]
+  int G;
+};
+
+// One tile: up to WG bunches of one ring (one lane each) times RW_TILE_TAPS
+// consecutive taps of the resistive-wall sum (src/transform_weak.c:780-833).
+// Tap m of the bunch in bucket kb reads history slot u = kb + m (slot
+// (-rev h + u) mod n of the cyclic buffer, bucket u mod h); only m < h - kb
+// reads the row this turn's k_reduce writes, so the taps m >= h - kb can be
+// summed during the turn's map.  The tile's bunches share the span of slots
+// [kb_lo + m0, kb_hi + m1], staged once in LDS, so the loads per turn are the
+// history once per tile instead of once per bunch.  Each term is the
+// reference's fNp * dip * I / sqrt(t) (the division is kept); one plane per
+// sweep keeps the register count low.
+__device__ void rw_old_tile(const DevParams *__restrict__ Pr, const RwOld &RO, int t, const int *bucket,
+                            const TurnState *__restrict__ tsr, double *sm)
+{
+  const int4 T = RO.tiles[t];
+  const DevParams &P = Pr[T.x];
+  const int tid = threadIdx.x;
+  const int h = P.h, n = (P.Nmlt + 2) * h, M = (P.Nmlt + 1) * h;
+  const long long rev = tsr[T.x].rev;
+  const int m0 = 1 + T.w * RW_TILE_TAPS;
+  const int m1 = min(m0 + RW_TILE_TAPS - 1, M);
+  const int kb_lo = bucket[T.y], kb_hi = bucket[T.y + T.z - 1];
+  const int u0 = kb_lo + m0, span = kb_hi - kb_lo + m1 - m0 + 1;
+  double *lx = sm, *lz = lx + span, *lI = lz + span, *lst = lI + span;
+  int *lf = (int *)(lst + RW_TILE_TAPS);
+  const double *hist = RO.rwhist + P.rw_off;
+  const double *Ib = RO.Ibunch + P.bucket_off;
+  const int *fill = RO.nfFill + P.bucket_off;
+  const long long k0 = (-rev * h + u0) % n;
+  const int base = (int)(k0 < 0 ? k0 + n : k0);
+  for (int e = tid; e < span; e += WG) {
+    int idx = base + e;
+    if (idx >= n) idx -= n;
+    const int bk = (u0 + e) % h;
+    lx[e] = hist[2 * idx];
+    lz[e] = hist[2 * idx + 1];
+    lI[e] = Ib[bk];
+    lf[e] = fill[bk];
+  }
+  for (int i = tid; i <= m1 - m0; i += WG) lst[i] = RO.sqrt_tt[P.st_off + m0 + i];
+  __syncthreads();
+  if (tid >= T.z) return;
+  const int b = T.y + tid, kb = bucket[b];
+  const int mlo = max(m0, h - kb); // earlier rows only: m >= h - kb
+  const double fNp = P.fNp;
+#pragma unroll 1
+  for (int pl = 0; pl < 2; pl++) {
+    const double *lv = pl ? lz : lx;
+    double a = 0.0;
+#pragma unroll 1
+    for (int m = mlo; m <= m1; m++) {
+      const int e = kb + m - u0;
+      const double tm = fNp * lv[e] * lI[e] / lst[m - m0];
+      a += lf[e] ? tm : 0.0;
+    }
+    RO.rw_part[((size_t)T.w * RO.G + b) * 2 + pl] = a;
+  }
+}
+
 template <int MODE, bool QE, bool DIP, bool GEN, bool MON>
 __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams *__restrict__ Pr, Coords C,
                                                                 const long long *wg_first, const long long *wg_end,
@@ -427,10 +495,16 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
                                                                 const double *sfV, const double *sfH,
                                                                 const double *lr_wake, long long lr_plane_stride,
                                                                 const double *pending, double *partial, HistPtr H,
-                                                                const TurnState *__restrict__ tsr)
+                                                                const TurnState *__restrict__ tsr, RwOld RO)
 {
   extern __shared__ unsigned long long smem[];
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  if (MODE == MODE_TURN && (int)blockIdx.x < RO.nlead) {
+    // leading blocks: the resistive-wall taps of earlier turns' rows, dispatched
+    // first so that they finish early in the pass
+    if ((int)blockIdx.x < RO.ntile) rw_old_tile(Pr, RO, blockIdx.x, bucket, tsr, (double *)smem);
+    return;
+  }
+  const int wg = xcd_remap(blockIdx.x - RO.nlead, gridDim.x - RO.nlead);
   const int b = wg_bunch[wg];
   const int ring = bunch_ring[b];
   const DevParams &P = Pr[ring];
@@ -961,15 +1035,15 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
   }
 }
 
-// ------------------------------------------------------- k_rw_old, k_rw_fresh
+// ------------------------------------------------------------- k_rw_fresh, k_rw
 
 // Resistive-wall long-range kick of bunch b (src/transform_weak.c:780-833): the
 // sum over the taps m = 1 .. (Nmlt+1) h of the ring's dipole history, tap m
 // reading the bucket m places after b's.  Only the taps m < h - kb read the row
-// this turn's k_reduce writes; the others read rows of earlier turns.  So the sum
-// is split: k_rw_old adds the earlier rows' taps on a second stream while the
-// turn's map runs, k_rw_fresh adds the fresh row's taps after k_reduce and stores
-// the pending kick the next pass applies.
+// this turn's k_reduce writes; the others read rows of earlier turns.  So for
+// large rings the sum is split: k_turn's leading blocks add the earlier rows'
+// taps while the map runs (rw_old_tile), k_rw_fresh adds the fresh row's taps
+// after k_reduce and stores the pending kick the next pass applies.
 //
 // rw_taps: NT lanes sum taps [mlo, mhi]; the block total ends in (sx, sz) of lane 0.
 template <int NT, int UNR>
@@ -1068,67 +1142,6 @@ __device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_his
   if (init && P.fb_switch_off) ts->has_fb = 0;
 }
 
-// The resistive-wall taps of the rows of earlier turns (second stream, concurrent
-// with the turn's map; rev and the bucket currents change only at the turn's
-// end).  One block per tile: up to WG bunches of one ring (one lane each) times
-// RW_TILE_TAPS consecutive taps.  Tap m of the bunch in bucket kb reads history
-// slot u = kb + m (slot (-rev h + u) mod n of the cyclic buffer, bucket u mod h),
-// so the tile's bunches share the span of slots [kb_lo + m0, kb_hi + m1], staged
-// once in LDS; the loads per turn are the history once per tile instead of once
-// per bunch.  Each term is the reference's fNp * dip * I / sqrt(t) (the division
-// is kept); one plane per sweep keeps it to 32 VGPRs, so that its waves fit
-// beside k_turn's (4 waves of <= 120 VGPRs per SIMD).
-__global__ __launch_bounds__(WG) void k_rw_old(const DevParams *__restrict__ Pr, const int4 *tiles,
-                                               const double *rwhist, const double *sqrt_tt, const double *Ibunch,
-                                               const int *nfFill, const int *bucket,
-                                               const TurnState *__restrict__ tsr, double *rw_part, int G)
-{
-  extern __shared__ double sm[];
-  const int4 T = tiles[blockIdx.x]; // ring, first bunch, bunches, tap tile
-  const DevParams &P = Pr[T.x];
-  const int tid = threadIdx.x;
-  const int h = P.h, n = (P.Nmlt + 2) * h, M = (P.Nmlt + 1) * h;
-  const long long rev = tsr[T.x].rev;
-  const int m0 = 1 + T.w * RW_TILE_TAPS;
-  const int m1 = min(m0 + RW_TILE_TAPS - 1, M);
-  const int kb_lo = bucket[T.y], kb_hi = bucket[T.y + T.z - 1];
-  const int u0 = kb_lo + m0, span = kb_hi - kb_lo + m1 - m0 + 1;
-  double *lx = sm, *lz = lx + span, *lI = lz + span, *lst = lI + span;
-  int *lf = (int *)(lst + RW_TILE_TAPS);
-  const double *hist = rwhist + P.rw_off;
-  const double *Ib = Ibunch + P.bucket_off;
-  const int *fill = nfFill + P.bucket_off;
-  const long long k0 = (-rev * h + u0) % n;
-  const int base = (int)(k0 < 0 ? k0 + n : k0);
-  for (int e = tid; e < span; e += WG) {
-    int idx = base + e;
-    if (idx >= n) idx -= n;
-    const int bk = (u0 + e) % h;
-    lx[e] = hist[2 * idx];
-    lz[e] = hist[2 * idx + 1];
-    lI[e] = Ib[bk];
-    lf[e] = fill[bk];
-  }
-  for (int i = tid; i <= m1 - m0; i += WG) lst[i] = sqrt_tt[P.st_off + m0 + i];
-  __syncthreads();
-  if (tid >= T.z) return;
-  const int b = T.y + tid, kb = bucket[b];
-  const int mlo = max(m0, h - kb); // earlier rows only: m >= h - kb
-  const double fNp = P.fNp;
-#pragma unroll 1
-  for (int pl = 0; pl < 2; pl++) {
-    const double *lv = pl ? lz : lx;
-    double a = 0.0;
-#pragma unroll 1
-    for (int m = mlo; m <= m1; m++) {
-      const int e = kb + m - u0;
-      const double t = fNp * lv[e] * lI[e] / lst[m - m0];
-      a += lf[e] ? t : 0.0;
-    }
-    rw_part[((size_t)T.w * G + b) * 2 + pl] = a;
-  }
-}
-
 static size_t rw_old_lds(int span) { return (size_t)span * (3 * 8 + 4) + RW_TILE_TAPS * 8; }
 
 // Per bunch, after k_reduce: the taps of this turn's row (m < h - kb) plus the
@@ -1176,7 +1189,7 @@ __global__ __launch_bounds__(WG) void k_rw_fresh(const DevParams *__restrict__ P
   }
 }
 
-// Small rings (the map too short to hide k_rw_old): every tap in one pass, and
+// Small rings (the map too short to hide the earlier rows' taps): every tap in one pass, and
 // the ring's last block to finish ends the turn (every block of the ring has read
 // the turn state and the bucket currents before its increment).
 __global__ __launch_bounds__(RW_WG) void k_rw(const DevParams *__restrict__ Pr, const int *bunch_ring,
@@ -1710,14 +1723,12 @@ struct mbt_ring {
   double *fnp_out, *dipV_out, *dipH_out;
   int *trash_tot, *trash_pend;
   TurnState *ts;
-  int4 *rw_tiles;      // k_rw_old tiles: ring, first bunch, bunches, tap tile
+  int4 *rw_tiles;      // RW tiles of k_turn's leading blocks: ring, first bunch, bunches, tap tile
   double *rw_part;     // [tap tiles][G][2] partial sums of earlier turns' rows
   int rw_ntile;
   size_t rw_lds;
   unsigned *ring_done; // [R] blocks of the ring done with the turn's fused RW pass
-  int rw_split;        // RW sum split into k_rw_old (side stream) and k_rw_fresh
-  hipStream_t side;    // k_rw_old, concurrent with the turn's map
-  hipEvent_t ev_fork, ev_join;
+  int rw_split;        // RW sum split: earlier rows' taps in k_turn's leading blocks, then k_rw_fresh
   GraphEntry graphs[NGRAPH];
   int graph_next, use_graphs;
   LRConst *lc;
@@ -1892,9 +1903,6 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
   }
   for (int i = 0; i < NGRAPH; i++)
     if (r->graphs[i].exec) (void)hipGraphExecDestroy(r->graphs[i].exec);
-  if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
-  if (r->ev_join) (void)hipEventDestroy(r->ev_join);
-  if (r->side) (void)hipStreamDestroy(r->side);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   if (r->rh)
     for (int k = 0; k < r->R; k++) {
@@ -2075,7 +2083,7 @@ static int setup_rw_gemm(mbt_ring *r)
   return MBT_OK;
 }
 
-// Tiles of k_rw_old: per RW ring, groups of up to WG consecutive bunches times
+// RW tiles of k_turn's leading blocks: per RW ring, groups of up to WG consecutive bunches times
 // RW_TILE_TAPS consecutive taps.  Leaves rw_split off (the one-pass sum) when a
 // tile's slot span does not fit the LDS budget.
 static int setup_rw_split(mbt_ring *r)
@@ -2153,9 +2161,6 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   // a blocking stream: ordered after work on the null stream (allocation-time
   // memsets) like every copy the library makes
   CK(hipStreamCreate(&r->stream));
-  CK(hipStreamCreate(&r->side));
-  CK(hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming));
-  CK(hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming));
   const int Nc = cfgs[0]->Ncell, NcA = Nc > 0 ? Nc : 1;
   r->Nc = Nc;
   r->nrev_mon = (int)cfgs[0]->NrevMon;
@@ -2551,7 +2556,7 @@ extern "C" int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *s) { return
 typedef void (*turn_kernel_t)(const DevParams *, Coords, const long long *, const long long *, const int *,
                               const int *, const int *, const long long *, const double *, const double *,
                               const double *, const double *, const double *, long long, const double *, double *,
-                              HistPtr, const TurnState *);
+                              HistPtr, const TurnState *, RwOld);
 
 #define TK(Q, D, G, M) k_turn<MODE_TURN, Q, D, G, M>
 static const turn_kernel_t g_turn_kernels[16] = {
@@ -2578,9 +2583,17 @@ static int launch_turn_pass(mbt_ring *r, int mode)
     k = g_turn_kernels[(r->any_qe ? 8 : 0) + (r->any_dip ? 4 : 0) + (r->any_gen ? 2 : 0) + (mon ? 1 : 0)];
   }
   const long long stride = (long long)r->H * (r->Nc > 0 ? r->Nc : 1);
-  hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
+  // the turn pass also carries the resistive-wall tiles of earlier turns' rows
+  RwOld ro = {};
+  size_t lds = r->lds_turn;
+  if (mode == MODE_TURN && r->rw_split) {
+    ro = {r->rw_tiles, r->rw_ntile, (r->rw_ntile + 7) / 8 * 8, r->rwhist, r->sqrt_tt, r->Ibunch, r->nfFill,
+          r->rw_part, r->G};
+    if (r->rw_lds > lds) lds = r->rw_lds;
+  }
+  hipLaunchKernelGGL(k, dim3(r->nwg + ro.nlead), dim3(WG), lds, r->stream, r->P, r->C, r->wg_first, r->wg_end,
                      r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
-                     r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
+                     r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts, ro);
   CK(hipGetLastError());
   return MBT_OK;
 }
@@ -2696,14 +2709,6 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   const int lr_launch = r->sf_any && r->nres > 0;
   const int rw_split = r->rw_split;
   if (E) CK(hipEventRecord(E[0], r->stream));
-  if (rw_split) {
-    CK(hipEventRecord(r->ev_fork, r->stream));
-    CK(hipStreamWaitEvent(r->side, r->ev_fork, 0));
-    hipLaunchKernelGGL(k_rw_old, dim3(r->rw_ntile), dim3(WG), r->rw_lds, r->side, r->P, r->rw_tiles, r->rwhist,
-                       r->sqrt_tt, r->Ibunch, r->nfFill, r->bucket, r->ts, r->rw_part, r->G);
-    CK(hipGetLastError());
-    CK(hipEventRecord(r->ev_join, r->side));
-  }
   if ((rc = launch_turn_pass(r, MODE_TURN))) return rc;
   if (E) CK(hipEventRecord(E[1], r->stream));
   if ((rc = launch_reduce(r, 2))) return rc;
@@ -2727,7 +2732,6 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
     hipLaunchKernelGGL(k_end, dim3(r->R), dim3(64), 0, r->stream, r->P, r->ts, r->bunch_Ib, r->Ibunch, r->nfFill,
                        !lr_launch);
   } else if (rw_split) {
-    CK(hipStreamWaitEvent(r->stream, r->ev_join, 0));
     hipLaunchKernelGGL(k_rw_fresh, dim3(r->G), dim3(WG), 0, r->stream, r->P, r->bunch_ring, r->rwhist, r->sqrt_tt,
                        r->Ibunch, r->nfFill, r->bucket, r->ts, r->rw_part, r->G, r->pending);
     CK(hipGetLastError());
@@ -3050,7 +3054,7 @@ extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
         if (rep == 1) CK(hipEventRecord(e0, r->stream));
         hipLaunchKernelGGL(k, dim3(r->nwg), dim3(WG), r->lds_turn, r->stream, r->P, r->C, r->wg_first, r->wg_end,
                            r->wg_bunch, r->bunch_ring, r->bucket, r->bunch_first, r->shift, r->sfL, r->sfV, r->sfH,
-                           r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts);
+                           r->lr_wake, stride, r->pending, r->partial, r->Hs, r->ts, RwOld{});
         CK(hipGetLastError());
       }
       CK(hipEventRecord(e1, r->stream));

@@ -160,11 +160,10 @@ def test_qe_statistics_match_reference_stream(tmp_path):
 
 @pytest.mark.parametrize("name", ["wakes", "fromfile", "scan", "edge_single"])
 def test_rw_overlap_matches_oracle(tmp_path, name):
-    """The resistive-wall sum split over two streams (the taps of earlier turns'
-    rows beside the map, the fresh row's taps after the reduction; what rings of
-    >= 4,194,304 macroparticles run) gives the oracle's turns, and replaying the
-    turns as graphs (a fork and a join per turn) equals launching them eagerly
-    bit for bit."""
+    """The split resistive-wall sum (the taps of earlier turns' rows in the
+    leading blocks of the turn's map, the fresh row's taps after the reduction;
+    what rings of >= 4,194,304 macroparticles run) gives the oracle's turns, and
+    replaying the turns as graphs equals launching them eagerly bit for bit."""
     path = write_variant(name, str(tmp_path))
     print("case rw overlap", name, flush=True)
     cfg = m.Config(path, str(tmp_path) + "/")
