@@ -2646,18 +2646,27 @@ static int download_slots(mbt_ring *r, long long rev0, long n, double *stats, do
 {
   CK(hipStreamSynchronize(r->stream));
   const int G = r->G;
-  for (long t = 0; t < n; t++) {
-    int slot = (int)((rev0 + t) % STATS_TURNS);
+  // statistics and RF rows: one copy per contiguous run of ring-buffer slots (at
+  // most two per block), not one per turn
+  for (long t = 0; t < n;) {
+    const int slot = (int)((rev0 + t) % STATS_TURNS);
+    const long run = STATS_TURNS - slot < n - t ? STATS_TURNS - slot : n - t;
     if (stats)
-      CK(hipMemcpy(stats + (size_t)t * G * 12, r->stats + (size_t)slot * G * 12, (size_t)G * 12 * sizeof(double),
-                   hipMemcpyDeviceToHost));
-    if (ampinv && ((rev0 + t + 1) % r->nrev_mon == 0))
+      CK(hipMemcpy(stats + (size_t)t * G * 12, r->stats + (size_t)slot * G * 12,
+                   (size_t)run * G * 12 * sizeof(double), hipMemcpyDeviceToHost));
+    if (rf)
+      CK(hipMemcpy(rf + (size_t)2 * r->R * t, r->rfbuf + (size_t)2 * r->R * slot,
+                   (size_t)run * 2 * r->R * sizeof(double), hipMemcpyDeviceToHost));
+    t += run;
+  }
+  // ampinv rows exist for the monitor turns only
+  if (ampinv)
+    for (long t = 0; t < n; t++) {
+      if ((rev0 + t + 1) % r->nrev_mon != 0) continue;
+      const int slot = (int)((rev0 + t) % STATS_TURNS);
       CK(hipMemcpy(ampinv + (size_t)t * G * 6, r->amp + (size_t)slot * G * 6, (size_t)G * 6 * sizeof(double),
                    hipMemcpyDeviceToHost));
-    if (rf)
-      CK(hipMemcpy(rf + (size_t)2 * r->R * t, r->rfbuf + (size_t)2 * r->R * slot, 2 * r->R * sizeof(double),
-                   hipMemcpyDeviceToHost));
-  }
+    }
   return MBT_OK;
 }
 
