@@ -21,7 +21,7 @@
 //             turns' rows are summed by k_turn's leading blocks during the map
 //             and k_rw_fresh adds this turn's row after k_reduce; then the turn
 //             end.  For ensembles of rings sharing one tap table the sum is one
-//             MFMA GEMM instead (k_rwg_y, k_rwg_tile, k_rwg_sum, then k_end);
+//             MFMA GEMM instead (k_rwg_tile, k_rwg_sum, then k_end);
 //   k_lr_par  with long-range resonators: the phasor recurrence over all buckets
 //             and the RF-feedback samples (construct_wake_phasor
 //             src/transform_weak.c:961-1048), linear, so one block per resonator
@@ -1121,18 +1121,10 @@ __device__ void end_turn(const DevParams &P, TurnState *ts, double *bunch_Ib, do
 // RF feedback of one ring: rffb_calc_mean_voltage_phase (src/feedback_rf.c:21-34)
 // and the solve of src/worker_weakweak.c:268-278; the phase and voltage before
 // the solve are kept for the potentials output.
-__device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_hist, int init)
+// The solve from the sums of the sampled voltages vs and phases ps.
+__device__ void fb_apply(const DevParams &P, TurnState *ts, double vs, double ps, int init)
 {
-  ts->phai0_prev = ts->phai0;
-  ts->Vrf0_prev = ts->Vrf0;
-  if (!ts->has_fb) return;
-  const double *fbh = fb_hist + P.fbh_off;
   const int fb_len = P.fb_len;
-  double vs = 0, ps = 0;
-  for (int m = 0; m < fb_len; m++) {
-    vs += fbh[2 * m];
-    ps += fbh[2 * m + 1];
-  }
   double vmbar = vs / fb_len, phibar = ps / fb_len;
   vmbar = vmbar * P.E0 * 1.0e3;
   double num = P.vrf_design * sin(P.phi0_design) - vmbar * sin(phibar);
@@ -1140,6 +1132,20 @@ __device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_his
   ts->phai0 = ph;
   ts->Vrf0 = num / sin(ph);
   if (init && P.fb_switch_off) ts->has_fb = 0;
+}
+
+__device__ void fb_solve(const DevParams &P, TurnState *ts, const double *fb_hist, int init)
+{
+  ts->phai0_prev = ts->phai0;
+  ts->Vrf0_prev = ts->Vrf0;
+  if (!ts->has_fb) return;
+  const double *fbh = fb_hist + P.fbh_off;
+  double vs = 0, ps = 0;
+  for (int m = 0; m < P.fb_len; m++) {
+    vs += fbh[2 * m];
+    ps += fbh[2 * m + 1];
+  }
+  fb_apply(P, ts, vs, ps, init);
 }
 
 static size_t rw_old_lds(int span) { return (size_t)span * (3 * 8 + 4) + RW_TILE_TAPS * 8; }
@@ -1237,10 +1243,10 @@ __global__ __launch_bounds__(RW_WG) void k_rw(const DevParams *__restrict__ Pr, 
 // -- the sum of src/transform_weak.c:806-824 for every bunch of every ring; the
 // ring's fNp and RWconst scale the result.
 //
-// k_rwg_y forms Y (one lane per element, from the rings' cyclic dipole
-// histories and bucket currents).  k_rwg_tile: one block per (16-column tile,
-// K slice of `per` steps of 4); the block stages its slice of Y (rows of 16
-// consecutive doubles) and the tap weights its rows need in LDS, then each wave multiplies up to RWG_RT row tiles per pass out of LDS
+// k_rwg_tile: one block per (16-column tile, K slice of `per` steps of 4); the
+// block forms its slice of Y in LDS straight from the rings' cyclic dipole
+// histories and bucket currents (Y is never stored) and stages the tap weights
+// its rows need, then each wave multiplies up to RWG_RT row tiles per pass out of LDS
 // (MFMA 16x16x4: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]) and the
 // block writes its K slice's partial sums.  k_rwg_sum adds the slices in a fixed
 // order and stores the pending kicks.
@@ -1266,34 +1272,12 @@ struct RWGemm {
   int per, KS;  // K steps (of 4) per slice, slices
   const int *col_ring;     // [N/2] ring of each column pair
   const double *c;         // [M + 1] tap weights, c[0] unused
-  double *Y;               // [K][N] history columns of this turn
   double *part;            // [KS][HP][N] K-slice partial sums
   const int *bucket_bunch; // [H] global bucket -> bunch (-1 empty)
 };
 
 // LDS bytes of one k_rwg_tile block: Y slice [4 per][16] + taps [4 per + HP - 1]
 static inline size_t rwg_lds(int per, int HP) { return sizeof(double) * ((size_t)68 * per + HP); }
-
-__global__ __launch_bounds__(256) void k_rwg_y(RWGemm Gm, const DevParams *__restrict__ Pr,
-                                             const TurnState *__restrict__ tsr, const double *rwhist,
-                                             const double *Ibunch, const int *nfFill)
-{
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)Gm.K * Gm.N) return;
-  const int a = (int)(e / Gm.N), col = (int)(e % Gm.N);
-  const int ring = Gm.col_ring[col >> 1], pl = col & 1;
-  const DevParams &P = Pr[ring];
-  double v = 0.0;
-  if (a >= 1 && a < Gm.h + Gm.M) {
-    const int q = a % Gm.h;
-    if (nfFill[P.bucket_off + q]) {
-      long long idx = (-tsr[ring].rev * Gm.h + a) % Gm.n;
-      if (idx < 0) idx += Gm.n;
-      v = rwhist[P.rw_off + 2 * idx + pl] * Ibunch[P.bucket_off + q];
-    }
-  }
-  Gm.Y[e] = v;
-}
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__restrict__ Pr,
@@ -1309,10 +1293,36 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
   const int A0 = 4 * s0, nA = 4 * (s1 - s0);
   double *Yf = sm;                        // [nA][16]
   double *cs = sm + (size_t)64 * Gm.per;  // taps m = A0 - HP + 1 + e
-  // ---- stage the Y slice and the taps
+  __shared__ long long ybase[16];         // history slot of row A0, per column
+  __shared__ int yring[16];
+  // ---- stage the Y slice (formed here from the history: Y[a][col] = dipole of
+  // history slot (-rev h + a) mod n times the current of bucket a mod h, zero
+  // for empty buckets and rows outside 1 <= a < h + M) and the taps
+  if (tid < 16) {
+    const int j = tj * 16 + tid;
+    const int ring = j < Gm.N ? Gm.col_ring[j >> 1] : -1;
+    yring[tid] = ring;
+    if (ring >= 0) {
+      long long k0 = (-tsr[ring].rev * h + A0) % Gm.n;
+      ybase[tid] = k0 < 0 ? k0 + Gm.n : k0;
+    }
+  }
+  __syncthreads();
+  const int q0 = A0 % h;
   for (int e = tid; e < nA * 16; e += 256) {
-    const int t = e >> 4, j = tj * 16 + (e & 15);
-    Yf[e] = j < Gm.N ? Gm.Y[(size_t)(A0 + t) * Gm.N + j] : 0.0;
+    const int t = e >> 4, jl = e & 15, ring = yring[jl];
+    const int a = A0 + t;
+    double v = 0.0;
+    if (ring >= 0 && a >= 1 && a < h + M) {
+      const DevParams &P = Pr[ring];
+      const int q = (q0 + t) % h;
+      if (nfFill[P.bucket_off + q]) {
+        long long idx = ybase[jl] + t;
+        if (idx >= Gm.n) idx %= Gm.n; // t < 4 per may exceed n when h is small
+        v = rwhist[P.rw_off + 2 * idx + (jl & 1)] * Ibunch[P.bucket_off + q];
+      }
+    }
+    Yf[e] = v;
   }
   const int mlo = A0 - HP + 1;
   for (int e = tid; e < nA + HP - 1; e += 256) {
@@ -1642,36 +1652,54 @@ __global__ __launch_bounds__(256) void k_lr_par(const LRConst *lc, const DevPara
 
 // lr_wake[plane][global bucket][j] = sum over the ring's resonators of that
 // plane, in order, of their terms
-__global__ void k_lr_sum(const int *lr_first, const LRConst *lc, const double *lrc, const int *nfFill,
-                         const int *bucket_ring, int H, int Nc, double *lr_wake)
+__global__ void k_lr_sum(const int *lr_first, const LRConst *lc, const double *lrc, const DevParams *__restrict__ Pr,
+                         const int *bunch_ring, const int *bucket, int G, int Nc, int4 planes,
+                         long long plane_stride, double *lr_wake)
 {
-  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long per = (long long)H * Nc;
-  if (e >= 3 * per) return;
-  int plane = (int)(e / per);
-  long long qj = e % per;
-  int q = (int)(qj / Nc);
-  int j = (int)(qj % Nc);
-  if (!nfFill[q]) {
-    lr_wake[e] = 0.0;
-    return;
-  }
-  const int ring = bucket_ring[q];
+  // (plane among the planes.x active ones, bunch, slice): only the buckets of
+  // bunches are read by the kick; the rows of empty buckets and of planes
+  // without resonators stay zero from the allocation
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)G * Nc;
+  if (e >= planes.x * per) return;
+  const int pi = (int)(e / per);
+  const int plane = pi == 0 ? planes.y : (pi == 1 ? planes.z : planes.w);
+  const long long bj = e % per;
+  const int b = (int)(bj / Nc), j = (int)(bj % Nc);
+  const int ring = bunch_ring[b];
+  const int q = Pr[ring].bucket_off + bucket[b];
   double s = 0.0;
   // the ring's resonators are contiguous in lc (ring-major, then plane, then index)
   for (int r = lr_first[ring]; r < lr_first[ring + 1]; r++)
     if (lc[r].plane == plane) s += lrc[lc[r].lrc_off + (size_t)(q - lc[r].bucket_off) * Nc + j];
-  lr_wake[e] = s;
+  lr_wake[plane * plane_stride + (long long)q * Nc + j] = s;
 }
 
 // RF feedback, one lane per ring: rffb_calc_mean_voltage_phase
 // (src/feedback_rf.c:21-34) and the solve of src/worker_weakweak.c:268-278.
-__global__ void k_fb_solve(int nrings, const DevParams *__restrict__ Pr, TurnState *tsr, const double *fb_hist,
-                           int init)
+// One wave per ring: the samples summed by 64 lanes (fixed order), then the solve.
+__global__ __launch_bounds__(64) void k_fb_solve(int nrings, const DevParams *__restrict__ Pr, TurnState *tsr,
+                                                 const double *fb_hist, int init)
 {
-  int ring = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ring = blockIdx.x, lane = threadIdx.x;
   if (ring >= nrings) return;
-  fb_solve(Pr[ring], tsr + ring, fb_hist, init);
+  const DevParams &P = Pr[ring];
+  TurnState *ts = tsr + ring;
+  const int has = ts->has_fb;
+  if (lane == 0) {
+    ts->phai0_prev = ts->phai0;
+    ts->Vrf0_prev = ts->Vrf0;
+  }
+  if (!has) return;
+  const double *fbh = fb_hist + P.fbh_off;
+  double vs = 0.0, ps = 0.0;
+  for (int m = lane; m < P.fb_len; m += 64) {
+    vs += fbh[2 * m];
+    ps += fbh[2 * m + 1];
+  }
+  vs = wave_sum(vs);
+  ps = wave_sum(ps);
+  if (lane == 0) fb_apply(P, ts, vs, ps, init);
 }
 
 // =============================================================== host side
@@ -1704,7 +1732,7 @@ struct mbt_ring {
   int mixed;         // MBT_RING_MIXED: FP32 wake convolutions, FP32 hi/lo RW GEMM
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
-  double *rwg_c, *rwg_part, *rwg_Y;
+  double *rwg_c, *rwg_part;
   int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
   int chunk_items;   // macroparticles per lane of one work-group's chunk
   int any_qe, any_dip, any_gen, nrev_mon;
@@ -1735,6 +1763,7 @@ struct mbt_ring {
   int *lr_first; // [R+1] first resonator of each ring in lc
   int nres;
   int lr_hmax; // largest h among the resonators' rings
+  int4 lr_planes_active; // x: planes with a resonator in some ring, y..w: those planes
   double *phasor_end, *fb_hist;
   long long rw_total, st_total, fbh_total;
   size_t lds_turn, lds_red;
@@ -1894,7 +1923,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
                   r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
                   r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->rw_tiles, r->rw_part, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
-                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part, r->rwg_Y};
+                  r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
     if (ptrs[i]) (void)hipFree(ptrs[i]);
   if (r->evpool) {
@@ -2064,7 +2093,6 @@ static int setup_rw_gemm(mbt_ring *r)
   if (!rc) rc = dalloc(&r->rwg_bucket_bunch, r->H);
   if (!rc) rc = dalloc(&r->rwg_c, (size_t)g.M + 1);
   if (!rc) rc = dalloc(&r->rwg_part, (size_t)g.KS * g.HP * g.N);
-  if (!rc) rc = dalloc(&r->rwg_Y, (size_t)g.K * g.N);
   if (!rc && (hipMemcpy(r->rwg_col_ring, cr, sizeof(int) * nrw, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_bucket_bunch, bb, sizeof(int) * r->H, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(r->rwg_c, c, sizeof(double) * (g.M + 1), hipMemcpyHostToDevice) != hipSuccess ||
@@ -2078,7 +2106,6 @@ static int setup_rw_gemm(mbt_ring *r)
   g.bucket_bunch = r->rwg_bucket_bunch;
   g.c = r->rwg_c;
   g.part = r->rwg_part;
-  g.Y = r->rwg_Y;
   r->rwg_on = 1;
   return MBT_OK;
 }
@@ -2315,6 +2342,15 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     for (int p = 0; p < 3; p++)
       if (r->rh[k].lr_plane[p]) nres += r->rh[k].cfg.n_lr[p];
   LRConst *lc = (LRConst *)calloc(nres > 0 ? nres : 1, sizeof(LRConst));
+  {
+    int act[3] = {0, 0, 0}, na = 0, pl[3] = {0, 0, 0};
+    for (int k = 0; k < R; k++)
+      for (int p = 0; p < 3; p++)
+        if (r->rh[k].lr_plane[p] && r->rh[k].cfg.n_lr[p] > 0) act[p] = 1;
+    for (int p = 0; p < 3; p++)
+      if (act[p]) pl[na++] = p;
+    r->lr_planes_active = make_int4(na, pl[0], pl[1], pl[2]);
+  }
   int *lrf = (int *)calloc(R + 1, sizeof(int));
   long long lrc_total = 0;
   {
@@ -2618,10 +2654,11 @@ static int launch_lr(mbt_ring *r, int init)
     hipLaunchKernelGGL(k_lr, dim3((r->nres + 63) / 64), dim3(64), 0, r->stream, r->nres, r->lc, r->P, r->fnp_ring,
                        stride, r->Ibunch, r->nfFill, r->Nc, r->phasor_end, r->lrc, r->fb_hist, r->ts, init);
   CK(hipGetLastError());
-  if (!init) {
-    long long tot = 3LL * r->H * r->Nc;
-    hipLaunchKernelGGL(k_lr_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->lr_first, r->lc, r->lrc,
-                       r->nfFill, r->bucket_ring, r->H, r->Nc, r->lr_wake);
+  if (!init && r->lr_planes_active.x > 0) {
+    const long long tot = (long long)r->lr_planes_active.x * r->G * r->Nc;
+    hipLaunchKernelGGL(k_lr_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, r->stream, r->lr_first, r->lc,
+                       r->lrc, r->P, r->bunch_ring, r->bucket, r->G, r->Nc, r->lr_planes_active,
+                       (long long)r->H * r->Nc, r->lr_wake);
     CK(hipGetLastError());
   }
   return MBT_OK;
@@ -2629,7 +2666,7 @@ static int launch_lr(mbt_ring *r, int init)
 
 static int launch_fb_solve(mbt_ring *r, int init)
 {
-  hipLaunchKernelGGL(k_fb_solve, dim3((r->R + 63) / 64), dim3(64), 0, r->stream, r->R, r->P, r->ts, r->fb_hist, init);
+  hipLaunchKernelGGL(k_fb_solve, dim3(r->R), dim3(64), 0, r->stream, r->R, r->P, r->ts, r->fb_hist, init);
   CK(hipGetLastError());
   return MBT_OK;
 }
@@ -2723,9 +2760,6 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
   if ((rc = launch_reduce(r, 2))) return rc;
   if (E) CK(hipEventRecord(E[2], r->stream));
   if (r->rwg_on) {
-    const long long ny = (long long)r->rwg.K * r->rwg.N;
-    hipLaunchKernelGGL(k_rwg_y, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, r->stream, r->rwg, r->P, r->ts,
-                       r->rwhist, r->Ibunch, r->nfFill);
     const dim3 tg((r->rwg.N + 15) / 16, r->rwg.KS);
     const size_t lds = rwg_lds(r->rwg.per, r->rwg.HP);
     if (r->mixed)
