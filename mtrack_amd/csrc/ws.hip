@@ -163,21 +163,19 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 __device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double iax,
                                           double iaz, double xx, double yy, double &Ex, double &Ey)
 {
-  int k1x = (int)(xx * iax), k1z = (int)(yy * iaz);
-  const bool pz = yy >= 0, px = xx >= 0;
-  // the reference's two sign cases as selects (no divergence): towards +2 for
-  // a non-negative coordinate, towards -2 otherwise
-  int k2z = pz ? (k1z < 2 ? k1z + 1 : k1z) : (k1z > -2 ? k1z - 1 : k1z);
-  k1z = pz ? (k1z > 2 ? 2 : k1z) : (k1z < -2 ? -2 : k1z);
-  k2z = pz ? (k2z > 2 ? 2 : k2z) : (k2z < -2 ? -2 : k2z);
-  int k2x = px ? (k1x < 2 ? k1x + 1 : k1x) : (k1x > -2 ? k1x - 1 : k1x);
-  k1x = px ? (k1x > 2 ? 2 : k1x) : (k1x < -2 ? -2 : k1x);
-  k2x = px ? (k2x > 2 ? 2 : k2x) : (k2x < -2 ? -2 : k2x);
-  const int i1 = (k1x + 2) * 5 + k1z + 2, i2 = (k2x + 2) * 5 + k1z + 2;
-  const int i3 = (k1x + 2) * 5 + k2z + 2, i4 = (k2x + 2) * 5 + k2z + 2;
-  const double fx = (xx - k1x * aAx) * iax, fz = (yy - k1z * aAz) * iaz;
-  // a - f d == a + (-f) d exactly, so the signed fraction reproduces both cases
-  const double sx = px ? fx : -fx, sz = pz ? fz : -fz;
+  // The reference's two sign cases are mirror images: a non-negative coordinate
+  // interpolates from cell k1 = trunc(x / d) towards +2, a negative one from k1
+  // towards -2, both clamped to the grid.  So the cell is found on |x| (trunc is
+  // sign-symmetric, as is every rounding below) and mirrored: with
+  // a = min(trunc(|x| / d), 2), k1 = s a and k2 = s min(a + 1, 2) for the sign
+  // s, and the signed fraction s (x - k1 d) / d equals (|x| - a d) / d exactly.
+  const int ax = min((int)(fabs(xx) * iax), 2), az = min((int)(fabs(yy) * iaz), 2);
+  const int bx = min(ax + 1, 2), bz = min(az + 1, 2);
+  const bool px = xx >= 0, pz = yy >= 0;
+  const int x1 = px ? 2 + ax : 2 - ax, x2 = px ? 2 + bx : 2 - bx;
+  const int z1 = pz ? 2 + az : 2 - az, z2 = pz ? 2 + bz : 2 - bz;
+  const int i1 = x1 * 5 + z1, i2 = x2 * 5 + z1, i3 = x1 * 5 + z2, i4 = x2 * 5 + z2;
+  const double sx = (fabs(xx) - ax * aAx) * iax, sz = (fabs(yy) - az * aAz) * iaz;
   const double y5 = ey[i1] + sx * (ey[i2] - ey[i1]);
   const double y6 = ey[i3] + sx * (ey[i4] - ey[i3]);
   const double x5 = ex[i1] + sx * (ex[i2] - ex[i1]);
