@@ -107,11 +107,36 @@ struct WSD {
 
 // ------------------------------------------------------------ device helpers
 
+// A double from another lane of the same row of 16 by a DPP move (no LDS
+// permute): CTRL 0x140 = row_mirror (lane 15 - i), 0x141 = row_half_mirror
+// (lane 7 - i within each 8), 0x4E = quad_perm [2,3,0,1], 0xB1 = quad_perm
+// [1,0,3,2].  Every partner is an involution, so both lanes of a pair form the
+// same sum and a butterfly over these four steps leaves the same total in all
+// 16 lanes, as the xor butterfly does.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the row of 16 lanes, in every lane of the row.
+__device__ __forceinline__ double rowsum16(double v)
+{
+  v += dpp_d<0x140>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0xB1>(v);
+  return v;
+}
+
 __device__ __forceinline__ double wsum64(double v)
 {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += __shfl_xor(v, 32, 64);
+  v += __shfl_xor(v, 16, 64);
+  return rowsum16(v);
 }
 
 // Eight sums over the 64 lanes of a wave at once: three halving exchanges (xor
@@ -133,10 +158,12 @@ __device__ __forceinline__ double wsum64x8(const double v[8])
     const double mine = b4 ? w[2 + k] : w[k], other = b4 ? w[k] : w[2 + k];
     x[k] = mine + __shfl_xor(other, 16, 64);
   }
-  double y = (b3 ? x[1] : x[0]) + __shfl_xor(b3 ? x[0] : x[1], 8, 64);
-  y += __shfl_xor(y, 4, 64);
-  y += __shfl_xor(y, 2, 64);
-  y += __shfl_xor(y, 1, 64);
+  // bit 3 by the row mirror (its partner 15 - i has bit 3 flipped), then the
+  // remaining three bits within each half row
+  double y = (b3 ? x[1] : x[0]) + dpp_d<0x140>(b3 ? x[0] : x[1]);
+  y += dpp_d<0x141>(y);
+  y += dpp_d<0x4E>(y);
+  y += dpp_d<0xB1>(y);
   return y;
 }
 
