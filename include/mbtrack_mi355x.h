@@ -252,15 +252,20 @@ double mbt_ring_bytes_per_turn(const mbt_ring *r);
 int mbt_ring_set_timing(mbt_ring *r, int on);
 long mbt_ring_turn(const mbt_ring *r);
 /* Work decomposition of the per-macroparticle pass: chunks of 256 * items
- * macroparticles of one bunch per work-group, items in [4, 64] (default 8).
- * No counterpart in the reference (a tuning knob of this build); changes results
- * only at the rounding level (grouping of the moment sums). */
+ * macroparticles of one bunch per work-group, items in [1, 64] for rings (all
+ * rings of an ensemble together) of fewer than 4,194,304 macroparticles and in
+ * [4, 64] otherwise.  The default is 8, halved while that leaves fewer than
+ * 2,048 work-groups.  No counterpart in the reference (a tuning knob of this
+ * build); changes results only at the rounding level (grouping of the moment
+ * sums). */
 int mbt_ring_set_chunk(mbt_ring *r, int items);
 int mbt_ring_get_chunk(const mbt_ring *r);
-/* Times the ring's own per-macroparticle turn kernel for items = 4, 8, 16, 32
- * (ms[4], may be NULL) -- the map, kicks, histogram and moment sums computed,
- * the coordinates written back unchanged, so the ring's state is untouched --
- * keeps the fastest and returns it in *best_items.  Call after mbt_ring_prepare. */
+/* Times the ring's own per-macroparticle turn kernel for items = 1, 2, 4, 8,
+ * 16, 32 (1 and 2 only below 4,194,304 macroparticles) -- the map, kicks,
+ * histogram and moment sums computed, the coordinates written back unchanged,
+ * so the ring's state is untouched -- keeps the fastest and returns it in
+ * *best_items; ms[6] (may be NULL) receives each candidate's time, 0 for those
+ * not tried.  Call after mbt_ring_prepare. */
 int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
 /* No counterpart in the reference (a measurement aid): the rate in GB/s of a
  * plain in-place update of the ring's own six coordinate arrays (96 bytes per

@@ -67,8 +67,14 @@
 #endif
 #define STATS_TURNS 256
 #define MIN_CHUNK_ITEMS 4
+// rings below this many macroparticles may use 1 or 2 items per lane (their
+// pass is latency-bound: fewer items per lane, more work-groups in flight);
+// larger rings keep the buffers of 4 items per lane
+#define SMALL_RING_MP (1LL << 22)
+// work-groups the default decomposition aims for (eight per CU)
+#define DEFAULT_WG_TARGET 2048
 #define RWG_MIN_RINGS 8
-#define AUTOTUNE_CANDIDATES 4
+#define AUTOTUNE_CANDIDATES 6
 #define AUTOTUNE_ROUNDS 3
 #define AUTOTUNE_REPS 2
 #define MAX_CHUNK_ITEMS 64
@@ -1735,6 +1741,7 @@ struct mbt_ring {
   double *rwg_c, *rwg_part;
   int nwg, nwg_cap;  // work-groups of the current decomposition, buffer capacity
   int chunk_items;   // macroparticles per lane of one work-group's chunk
+  int min_items;     // smallest chunk the buffers allow (1 for small rings, else 4)
   int any_qe, any_dip, any_gen, nrev_mon;
   int sf_any;
   Coords C;
@@ -2005,8 +2012,8 @@ static void drop_graphs(mbt_ring *r)
 // WG * items macroparticles, one work-group each.
 static int set_decomposition(mbt_ring *r, int items)
 {
-  if (items < MIN_CHUNK_ITEMS || items > MAX_CHUNK_ITEMS) {
-    mbt_set_error("chunk of %d items per lane outside [%d, %d]", items, MIN_CHUNK_ITEMS, MAX_CHUNK_ITEMS);
+  if (items < r->min_items || items > MAX_CHUNK_ITEMS) {
+    mbt_set_error("chunk of %d items per lane outside [%d, %d]", items, r->min_items, MAX_CHUNK_ITEMS);
     return MBT_EINVAL;
   }
   const int nwg = wg_count(r, items), G = r->G;
@@ -2247,7 +2254,8 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   r->fbh_total = fbh_total;
   // ---- workgroup decomposition (contiguous chunks of one bunch): buffers sized
   // for the smallest chunk, the decomposition itself set below
-  const int nwg = wg_count(r, MIN_CHUNK_ITEMS);
+  r->min_items = Ntot < SMALL_RING_MP ? 1 : MIN_CHUNK_ITEMS;
+  const int nwg = wg_count(r, r->min_items);
   r->nwg_cap = nwg;
   int *nph = (int *)malloc(G * sizeof(int)), *bring = (int *)malloc(G * sizeof(int)), *bk = (int *)malloc(G * sizeof(int));
   long long *bfirst = (long long *)malloc(G * sizeof(long long));
@@ -2521,7 +2529,12 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     mbt_ring_destroy(r);
     return MBT_EDEVICE;
   }
-  int rcd = set_decomposition(r, CHUNK_ITEMS);
+  // default chunk: CHUNK_ITEMS, fewer items per lane where the ring has too few
+  // macroparticles to give DEFAULT_WG_TARGET work-groups
+  int items0 = CHUNK_ITEMS;
+  while (items0 > r->min_items && wg_count(r, items0) < DEFAULT_WG_TARGET) items0 /= 2;
+  if (items0 < r->min_items) items0 = r->min_items;
+  int rcd = set_decomposition(r, items0);
   if (!rcd && !(flags & MBT_RING_NO_RW_GEMM)) rcd = setup_rw_gemm(r);
   r->rw_split = !r->rwg_on && r->rw_total > 0 && (r->Ntot >= RW_SPLIT_MIN_MP || (flags & MBT_RING_RW_OVERLAP));
   if (!rcd && r->rw_split) rcd = setup_rw_split(r);
@@ -3074,7 +3087,7 @@ extern "C" int mbt_ring_get_chunk(const mbt_ring *r) { return r->chunk_items; }
 // how the moment sums are grouped, i.e. results at the rounding level.
 extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
 {
-  static const int cand[AUTOTUNE_CANDIDATES] = {4, 8, 16, 32};
+  static const int cand[AUTOTUNE_CANDIDATES] = {1, 2, 4, 8, 16, 32};
   CK(hipSetDevice(r->device));
   (void)hipGetLastError();
   hipEvent_t e0, e1;
@@ -3091,6 +3104,7 @@ extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
   for (int c = 0; c < AUTOTUNE_CANDIDATES; c++) tbest[c] = 1e300;
   for (int round = 0; round < AUTOTUNE_ROUNDS && rc == MBT_OK; round++)
     for (int c = 0; c < AUTOTUNE_CANDIDATES && rc == MBT_OK; c++) {
+      if (cand[c] < r->min_items) continue; // below the buffers of a large ring
       if ((rc = set_decomposition(r, cand[c]))) break;
       float t = 0.0f;
       for (int rep = 0; rep <= AUTOTUNE_REPS; rep++) {
@@ -3108,7 +3122,7 @@ extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
     }
   double best_ms = 1e300;
   for (int c = 0; c < AUTOTUNE_CANDIDATES; c++) {
-    if (ms) ms[c] = tbest[c];
+    if (ms) ms[c] = tbest[c] < 1e300 ? tbest[c] : 0.0; // 0: not tried
     if (tbest[c] < best_ms) {
       best_ms = tbest[c];
       best = cand[c];

@@ -43,6 +43,8 @@ def test_ensemble_equals_single_rings(tmp_path):
     for k, cfg in enumerate(cfgs):
         print("member", MEMBERS[k], flush=True)
         g = m.Ring(cfg)
+        # the default chunk depends on the macroparticle count: use the ensemble's
+        g.set_chunk(ens.chunk())
         g0 = g.prepare()
         gst, gam, grf = g.track(TURNS)
         ghc, gtr = g.histograms()

@@ -104,7 +104,7 @@ def test_deterministic_bitwise(tmp_path):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("items", [4, 32])
+@pytest.mark.parametrize("items", [1, 4, 32])
 def test_chunk_sizes_match_oracle(tmp_path, items):
     """Any work decomposition of the per-macroparticle pass gives the oracle's
     turns to the same bar (only the grouping of the moment sums changes); the
@@ -118,7 +118,8 @@ def test_chunk_sizes_match_oracle(tmp_path, items):
     g = m.Ring(cfg)
     g.prepare()
     best, ms = g.autotune()
-    assert best in (4, 8, 16, 32) and all(v > 0 for v in ms.values())
+    assert best in (1, 2, 4, 8, 16, 32) and best in ms and all(v > 0 for v in ms.values())
+    assert set(ms) == {1, 2, 4, 8, 16, 32}  # a small ring tries the 1- and 2-item chunks too
     g.set_chunk(items)
     assert g.chunk() == items
     gst, _, _ = g.track(n)
