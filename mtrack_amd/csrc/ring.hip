@@ -428,7 +428,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int n)
 struct RwOld {
   const int4 *tiles; // ring, first bunch, bunches, tap tile
   int ntile, nlead;  // tiles; leading blocks (ntile rounded up to whole rounds of the 8 XCDs)
-  const double *rwhist, *sqrt_tt, *Ibunch;
+  const double *rwhist, *rsqrt_tt, *Ibunch; // rsqrt_tt: 1 / sqrt(m T0 / h) per tap
   const int *nfFill;
   double *rw_part; // [tap tiles][G][2]
   int G;
@@ -441,9 +441,10 @@ struct RwOld {
 // reads the row this turn's k_reduce writes, so the taps m >= h - kb can be
 // summed during the turn's map.  The tile's bunches share the span of slots
 // [kb_lo + m0, kb_hi + m1], staged once in LDS, so the loads per turn are the
-// history once per tile instead of once per bunch.  Each term is the
-// reference's fNp * dip * I / sqrt(t) (the division is kept); one plane per
-// sweep keeps the register count low.
+// history once per tile instead of once per bunch.  Each term is fNp * dip * I
+// times the tabulated 1 / sqrt(t) (the reference divides by sqrt(t): one
+// rounding apart, and a multiply instead of a division keeps these blocks short
+// beside the map's); one plane per sweep keeps the register count low.
 __device__ void rw_old_tile(const DevParams *__restrict__ Pr, const RwOld &RO, int t, const int *bucket,
                             const TurnState *__restrict__ tsr, double *sm)
 {
@@ -472,7 +473,7 @@ __device__ void rw_old_tile(const DevParams *__restrict__ Pr, const RwOld &RO, i
     lI[e] = Ib[bk];
     lf[e] = fill[bk];
   }
-  for (int i = tid; i <= m1 - m0; i += WG) lst[i] = RO.sqrt_tt[P.st_off + m0 + i];
+  for (int i = tid; i <= m1 - m0; i += WG) lst[i] = RO.rsqrt_tt[P.st_off + m0 + i];
   __syncthreads();
   if (tid >= T.z) return;
   const int b = T.y + tid, kb = bucket[b];
@@ -485,7 +486,7 @@ __device__ void rw_old_tile(const DevParams *__restrict__ Pr, const RwOld &RO, i
 #pragma unroll 1
     for (int m = mlo; m <= m1; m++) {
       const int e = kb + m - u0;
-      const double tm = fNp * lv[e] * lI[e] / lst[m - m0];
+      const double tm = fNp * lv[e] * lI[e] * lst[m - m0];
       a += lf[e] ? tm : 0.0;
     }
     RO.rw_part[((size_t)T.w * RO.G + b) * 2 + pl] = a;
@@ -1753,7 +1754,7 @@ struct mbt_ring {
   RedOut Ro;
   double *sfL, *sfV, *sfH, *lr_wake, *fnp_ring, *lrc;
   double *Gf;
-  double *pending, *rwhist, *sqrt_tt, *Ibunch, *bunch_Ib;
+  double *pending, *rwhist, *sqrt_tt, *rsqrt_tt, *Ibunch, *bunch_Ib;
   double *stats, *amp, *rfbuf;
   double *fnp_out, *dipV_out, *dipH_out;
   int *trash_tot, *trash_pend;
@@ -1928,7 +1929,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
                   r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
                   r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
-                  r->rwhist, r->sqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
+                  r->rwhist, r->sqrt_tt, r->rsqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
                   r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->rw_tiles, r->rw_part, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
                   r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++)
@@ -2331,6 +2332,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   DA(r->pending, (size_t)G * 2);
   DA(r->rwhist, rw_total > 0 ? rw_total : 2);
   DA(r->sqrt_tt, st_total > 0 ? st_total : 1);
+  DA(r->rsqrt_tt, st_total > 0 ? st_total : 1);
   DA(r->Ibunch, H);
   DA(r->bunch_Ib, G);
   DA(r->stats, (size_t)STATS_TURNS * G * 12);
@@ -2446,6 +2448,9 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
       for (long long m = 1; m <= M; m++) s[m] = sqrt(m * deltab);
     }
     UP(r->sqrt_tt, st, st_total);
+    CK(hipStreamSynchronize(r->stream));
+    for (long long i = 0; i < st_total; i++) st[i] = 1.0 / st[i];
+    UP(r->rsqrt_tt, st, st_total);
     CK(hipStreamSynchronize(r->stream));
     free(st);
   }
@@ -2636,7 +2641,7 @@ static int launch_turn_pass(mbt_ring *r, int mode)
   RwOld ro = {};
   size_t lds = r->lds_turn;
   if (mode == MODE_TURN && r->rw_split) {
-    ro = {r->rw_tiles, r->rw_ntile, (r->rw_ntile + 7) / 8 * 8, r->rwhist, r->sqrt_tt, r->Ibunch, r->nfFill,
+    ro = {r->rw_tiles, r->rw_ntile, (r->rw_ntile + 7) / 8 * 8, r->rwhist, r->rsqrt_tt, r->Ibunch, r->nfFill,
           r->rw_part, r->G};
     if (r->rw_lds > lds) lds = r->rw_lds;
   }
