@@ -261,10 +261,10 @@ long mbt_ring_turn(const mbt_ring *r);
 int mbt_ring_set_chunk(mbt_ring *r, int items);
 int mbt_ring_get_chunk(const mbt_ring *r);
 /* Times the ring's own per-macroparticle turn kernel for items = 1, 2, 4, 8,
- * 16, 32 (1 and 2 only below 4,194,304 macroparticles) -- the map, kicks,
+ * 16, 32, 64 (1 and 2 only below 4,194,304 macroparticles) -- the map, kicks,
  * histogram and moment sums computed, the coordinates written back unchanged,
  * so the ring's state is untouched -- keeps the fastest and returns it in
- * *best_items; ms[6] (may be NULL) receives each candidate's time, 0 for those
+ * *best_items; ms[7] (may be NULL) receives each candidate's time, 0 for those
  * not tried.  Call after mbt_ring_prepare. */
 int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
 /* No counterpart in the reference (a measurement aid): the rate in GB/s of a

@@ -230,9 +230,9 @@ class Ring:
     def autotune(self):
         """Chooses the per-work-group chunk by timing; returns (items, {items: ms})."""
         best = ctypes.c_int()
-        ms = np.zeros(6)
+        ms = np.zeros(7)
         _check(lib().mbt_ring_autotune(self.ptr, ctypes.byref(best), dptr(ms)))
-        return best.value, {k: v for k, v in zip((1, 2, 4, 8, 16, 32), ms.tolist()) if v > 0}
+        return best.value, {k: v for k, v in zip((1, 2, 4, 8, 16, 32, 64), ms.tolist()) if v > 0}
 
     def stream_ceiling(self, reps=10):
         """GB/s of a plain in-place update of the ring's own coordinate arrays

@@ -74,7 +74,7 @@
 // work-groups the default decomposition aims for (eight per CU)
 #define DEFAULT_WG_TARGET 2048
 #define RWG_MIN_RINGS 8
-#define AUTOTUNE_CANDIDATES 6
+#define AUTOTUNE_CANDIDATES 7
 #define AUTOTUNE_ROUNDS 3
 #define AUTOTUNE_REPS 2
 #define MAX_CHUNK_ITEMS 64
@@ -3092,7 +3092,7 @@ extern "C" int mbt_ring_get_chunk(const mbt_ring *r) { return r->chunk_items; }
 // how the moment sums are grouped, i.e. results at the rounding level.
 extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
 {
-  static const int cand[AUTOTUNE_CANDIDATES] = {1, 2, 4, 8, 16, 32};
+  static const int cand[AUTOTUNE_CANDIDATES] = {1, 2, 4, 8, 16, 32, 64};
   CK(hipSetDevice(r->device));
   (void)hipGetLastError();
   hipEvent_t e0, e1;
