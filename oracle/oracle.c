@@ -18,6 +18,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
@@ -411,6 +412,80 @@ static void greens(orc_ring *r)
   }
 }
 
+/* construct_greensfunc_file, src/transform_weak.c:406-471.  Each file's
+ * samples (t [ns], w) are read whole first; the reader then starts at the first
+ * sample with t >= 0 and walks the bin edges k dTau as the reference does: for
+ * bin i it advances to the first sample beyond i dTau (remembering the last one
+ * at or before it), skips the bins that sample also lies beyond, and fills the
+ * bins since the last filled one by the linear interpolation between the two
+ * samples.  After the last sample the reference's reader keeps the last pair
+ * for both ends (0 / 0 in the interpolation); bins beyond Ncell that its walk
+ * would write are not written here.  Longitudinal functions get their first
+ * bin halved. */
+static int greens_file(orc_ring *r)
+{
+  const mbt_config *c = &r->c;
+  const int Nc = r->Nc;
+  const double dTau = c->sigma_tau * c->dT;
+  for (int j = 0; j < c->n_wakefile; j++) {
+    FILE *fp = fopen(c->wakefile[j], "r");
+    if (!fp) return -1;
+    size_t n = 0, cap = 1024;
+    double *ts = malloc(cap * sizeof(double)), *ws = malloc(cap * sizeof(double)), a, b;
+    while (fscanf(fp, "%lf %lf", &a, &b) == 2) {
+      if (n == cap) {
+        cap *= 2;
+        ts = realloc(ts, cap * sizeof(double));
+        ws = realloc(ws, cap * sizeof(double));
+      }
+      ts[n] = a;
+      ws[n] = b;
+      n++;
+    }
+    fclose(fp);
+    double *G = calloc(Nc, sizeof(double));
+    size_t p = 0;
+    double t = 0.0, w = 0.0;
+    while (p < n) {
+      t = ts[p];
+      w = ws[p];
+      p++;
+      if (!(t < 0)) break;
+    }
+    double lt = 0.0, lw = 0.0;
+    long last = -1;
+    int eof = 0;
+    for (long i = 0; i < Nc; i++) {
+      if (eof) G[i] = G[i - 1];
+      while (t <= i * dTau) {
+        lt = t;
+        lw = w;
+        if (p >= n) {
+          eof = 1;
+          break;
+        }
+        t = ts[p];
+        w = ws[p];
+        p++;
+      }
+      while (t > (i + 1) * dTau) i++;
+      for (long k = last + 1; k <= i && k < Nc; k++) G[k] = (w * (k * dTau - lt) + lw * (t - k * dTau)) / (t - lt);
+      if (i > last + 1) i--;
+      last = i;
+    }
+    double *dst = c->wakefile_plane[j] == LON ? r->Gl : c->wakefile_plane[j] == HOR ? r->Gh : r->Gv;
+    if (c->wakefile_plane[j] == LON) G[0] = G[0] / 2.;
+    for (int i = 0; i < Nc; i++) dst[i] += G[i];
+    if (c->wakefile_plane[j] == LON) r->PlaneL++;
+    else if (c->wakefile_plane[j] == HOR) r->PlaneH++;
+    else r->PlaneV++;
+    free(G);
+    free(ts);
+    free(ws);
+  }
+  return 0;
+}
+
 /* RW_table_LON / RW_table_TRANS, src/rw_table.c:311-395, on the tabulated
  * first-bin values (data shared with the product: the reference gives no
  * closed form for them) */
@@ -494,7 +569,6 @@ int orc_ring_greens(const orc_ring *r, double *Gl, double *Gv, double *Gh, int *
 
 orc_ring *orc_ring_create(const mbt_config *c, int rng_mode)
 {
-  if (c->n_wakefile > 0) return NULL;
   orc_ring *r = calloc(1, sizeof *r);
   r->c = *c;
   r->rng_mode = rng_mode;
@@ -533,7 +607,7 @@ orc_ring *orc_ring_create(const mbt_config *c, int rng_mode)
   r->Gh = calloc(Nc, sizeof(double));
   if (r->Nc > 0) {
     greens(r);
-    if ((c->EnableRW_short || c->EnableRW_short_LON) && greens_rw(r) < 0) {
+    if (greens_file(r) < 0 || ((c->EnableRW_short || c->EnableRW_short_LON) && greens_rw(r) < 0)) {
       orc_ring_destroy(r);
       return NULL;
     }

@@ -210,6 +210,16 @@ int main(int argc, char **argv)
     if (w->ring.resonators_size > 0) construct_greensfunc_resonator(&w->sf, &w->ring);
     if (w->ring.wakefiles_size > 0) construct_greensfunc_file(&w->sf, &w->ring);
     if (track.EnableRW_short || track.EnableRW_short_LON) construct_greensfunc_RW(&w->sf, &w->ring);
+    if (k == 0 && w->sf.Ncell > 0) {
+      /* the reference's Green functions (identical for every bunch) */
+      snprintf(path, sizeof path, "%s/greens.txt", track.work_path);
+      FILE *fg = fopen(path, "w");
+      if (!fg) return 1;
+      for (int i = 0; i < w->sf.Ncell; i++)
+        fprintf(fg, "%d %.17g %.17g %.17g\n", i, w->sf.Gl[i], w->sf.Gv[i], w->sf.Gh[i]);
+      fprintf(fg, "planes %d %d %d\n", w->sf.PlaneL, w->sf.PlaneH, w->sf.PlaneV);
+      fclose(fg);
+    }
     weak_bunch_calc_statistics(&w->bunch, &track);
     dump_stats(fst, -1, w->kb, &w->bunch.stats);
     if (w->bunch.kb_out == 1 && track.NrevPotentialsOut <= track.NrevTot) {

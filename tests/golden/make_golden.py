@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from variants import VARIANTS, write_variant  # noqa: E402
+from variants import VARIANTS, conf_template, write_variant  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "refharness")
 
@@ -41,7 +41,7 @@ def load_params(path):
 def make(name):
     with tempfile.TemporaryDirectory() as d:
         conf = write_variant(name, d)
-        text = open(conf).read().replace(os.path.abspath(os.path.join(d, "fill.txt")), "@FILL@")
+        text = conf_template(open(conf).read(), d)
         first = None
         r = subprocess.run([HARNESS, conf, d, "0", "1", "-1"], capture_output=True, text=True)
         if r.returncode:
@@ -61,9 +61,15 @@ def make(name):
             hist = hist[keep]
         parts = np.loadtxt(os.path.join(d, "parts_%d.txt" % first), ndmin=2)
         pn, pv = load_params(os.path.join(d, "params.txt"))
+        gp = os.path.join(d, "greens.txt")
+        greens, planes = np.zeros((0, 4)), np.zeros(3, dtype=int)
+        if os.path.exists(gp):
+            lines = open(gp).read().split("\n")
+            greens = np.array([[float(v) for v in l.split()] for l in lines if l and not l.startswith("planes")])
+            planes = np.array([int(v) for v in [l for l in lines if l.startswith("planes")][0].split()[1:]])
     out = os.path.join(HERE, name + ".npz")
     np.savez_compressed(out, conf=np.array(text), stats=st, rf=rf, ampinv=am, hist=hist, parts=parts,
-                        param_names=pn, param_values=pv)
+                        param_names=pn, param_values=pv, greens=greens, greens_planes=planes)
     return out
 
 
@@ -86,7 +92,7 @@ def make_inp():
 
 # per-variant section overrides of the end-to-end cases (none at present)
 E2E_OVERRIDES = {}
-E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform", "rw_short"]
+E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform", "rw_short", "wakefile"]
 
 MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"

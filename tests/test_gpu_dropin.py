@@ -68,7 +68,7 @@ def compare_columns(a_text, b_text, rel):
     return ["column %d: %.3g > %g" % (c, e, rel) for c, e in enumerate(err) if e > rel]
 
 
-@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform", "rw_short"])
+@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform", "rw_short", "wakefile"])
 def test_cli_files_match_reference(tmp_path, case):
     gold = os.path.join(GOLD_ROOT, "e2e_" + case)
     d = str(tmp_path)

@@ -16,7 +16,7 @@ import pytest
 
 import mtrack_amd as m
 import oracle as o
-from variants import VARIANTS, write_variant
+from variants import VARIANTS, conf_template, write_variant
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -39,7 +39,7 @@ def oracle_run(path, work, rng=o.RNG_REFERENCE):
 def test_oracle_bitwise_vs_reference(tmp_path, name):
     g = fixture(name)
     path = write_variant(name, str(tmp_path))
-    text = open(path).read().replace(os.path.abspath(os.path.join(str(tmp_path), "fill.txt")), "@FILL@")
+    text = conf_template(open(path).read(), str(tmp_path))
     assert text == str(g["conf"]), "variants.py changed: regenerate tests/golden"
     cfg, r, st0, st, am, rf, fin = oracle_run(path, str(tmp_path) + "/")
     nb = cfg.geti("Nbunch")
@@ -130,3 +130,22 @@ def test_greens_product_equals_oracle(tmp_path, name):
     ol, ov, oh, oplanes = orc.greens()
     assert np.array_equal(gl, ol) and np.array_equal(gv, ov) and np.array_equal(gh, oh), name
     assert planes == oplanes
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_oracle_greens_bitwise_vs_reference(tmp_path, name):
+    """The oracle's Green functions (resonators, wake files, short-range
+    resistive wall) equal the reference's own (selffield_model_t Gl/Gv/Gh after
+    construct_greensfunc_resonator / _file / _RW, dumped by the reference
+    harness) bit for bit, and so do the plane counters."""
+    g = fixture(name)
+    path = write_variant(name, str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    if cfg.geti("Ncell") <= 0:
+        pytest.skip("no selffield")
+    ol, ov, oh, planes = o.OracleRing(cfg).greens()
+    ref = g["greens"]
+    assert ref.shape == (cfg.geti("Ncell"), 4)
+    assert np.all(np.isfinite(ref[:, 1:]))
+    assert np.array_equal(ol, ref[:, 1]) and np.array_equal(ov, ref[:, 2]) and np.array_equal(oh, ref[:, 3]), name
+    assert planes == list(g["greens_planes"])

@@ -86,9 +86,51 @@ VARIANTS = {
                                  "ring": _ring(), "distribution": {"Np": 999}},
                        extra_sections=[BBR_L, ("longrange_resonator_1", {"plane": "LON", "fres": 352.5, "Rs": 1e6,
                                                                          "Qfactor": 3e3})]),
+    # wake functions read from files (construct_greensfunc_file): a longitudinal
+    # file in the LON plane, a transverse one in VER and HOR, each sampled finely,
+    # then coarsely (steps of about 2.5 bins, the branch that fills several bins
+    # from one pair of samples), then finely again past the last bin; a broadband
+    # resonator adds to the longitudinal function
+    "wakefile": dict(sections={"selffield": {"Ncell": 50, "Nsigma": 5.0},
+                               "tracking": {"NrevTot": 30}, "ring": _ring(Iring=60.0),
+                               "distribution": {"Np": 400}},
+                     extra_sections=[BBR_L,
+                                     ("wake_file_1", {"filename": "@WAKE_L@", "plane": "LON"}),
+                                     ("wake_file_2", {"filename": "@WAKE_T@", "plane": "VER"}),
+                                     ("wake_file_3", {"filename": "@WAKE_T@", "plane": "HOR"})]),
 }
 
 FILL_BUCKETS = [int(k * 6.5) for k in range(64)]
+WAKE_FILES = {"@WAKE_L@": "wake_lon.dat", "@WAKE_T@": "wake_trans.dat"}
+
+
+def wake_samples():
+    """Sample times [s] of the wake files (the reader compares them with the bin
+    edges k dTau in seconds): two negative times (skipped by the reader), 0 to
+    30 ps in steps of 0.71 ps, 30 to 72 ps in steps of 5.1 ps, then to 125 ps in
+    steps of 1.3 ps (bins of 2.0422 ps: 50 slices over +-5 sigma of
+    10.007 ps)."""
+    import numpy as np
+    return 1e-12 * np.concatenate([[-4.0, -1.0], np.arange(0.0, 30.0, 0.71), np.arange(30.0, 72.0, 5.1),
+                                   np.arange(72.0, 125.0, 1.3)])
+
+
+def write_wake_files(d):
+    """The two wake files of the "wakefile" variant in d: a damped cosine
+    (longitudinal, amplitude 1e14) and a damped sine (transverse, 3e15), in the
+    units of the Green functions they are added to."""
+    import numpy as np
+    t = wake_samples()
+    out = {}
+    for key, (name, amp, shape) in {"@WAKE_L@": ("wake_lon.dat", 1.0e14, np.cos),
+                                    "@WAKE_T@": ("wake_trans.dat", 3.0e15, np.sin)}.items():
+        w = amp * np.exp(-np.maximum(t, 0.0) / 20e-12) * shape(2 * np.pi * t / 30e-12)
+        path = os.path.join(d, name)
+        with open(path, "w") as f:
+            for a, b in zip(t, w):
+                f.write("%.10g %.10g\n" % (a, b))
+        out[key] = os.path.abspath(path)
+    return out
 
 
 def write_variant(name, d, **section_overrides):
@@ -104,7 +146,19 @@ def write_variant(name, d, **section_overrides):
         secs.setdefault(sec, {}).update(keys)
     v["sections"] = secs
     text = cg.ring_conf(**v).replace("@FILL@", os.path.abspath(fill))
+    if "@WAKE_" in text:
+        for key, path in write_wake_files(d).items():
+            text = text.replace(key, path)
     return cg.write(os.path.join(d, name + ".conf"), text)
+
+
+def conf_template(text, d):
+    """A written variant's .conf text with the paths of its filling and wake files
+    in d replaced by their placeholders (the form the golden fixtures store)."""
+    text = text.replace(os.path.abspath(os.path.join(d, "fill.txt")), "@FILL@")
+    for key, name in WAKE_FILES.items():
+        text = text.replace(os.path.abspath(os.path.join(d, name)), key)
+    return text
 
 
 # ---- weak-strong (BunchModel = STRONG) configurations: slice-centroid bunches,
