@@ -44,6 +44,7 @@
 #define WS_HMAX 500     /* buckets of the CM history */
 #define WS_MSG 5000     /* doubles of the history message, src/manager_weakstrong.c:170 */
 #define WS_DATA 15000
+#define ORC_ION_BLOCK 4096 /* ions per block of the parallel train */
 
 /* physical constants of src/def.h:21-27 */
 #define K_C 2.997925e+08
@@ -76,6 +77,7 @@ struct orc_ws {
   long nF, capF;
   int kbF;
   double *NriF, *xxF, *xdF, *yyF, *ydF;
+  double *train_buf; /* [WS_NC][ORC_ION_BLOCK][2] Nri * E products of one block of ions */
   /* one bunch's fields and local ions */
   double Ex[WS_NC * WS_NG * WS_NG], Ey[WS_NC * WS_NG * WS_NG];
   double Nri[WS_NC * WS_NG * WS_NG], xxi[WS_NC * WS_NG * WS_NG], xdi[WS_NC * WS_NG * WS_NG];
@@ -179,6 +181,7 @@ void orc_ws_destroy(orc_ws *w)
   free(w->zzh);
   free(w->zph);
   free(w->NriF);
+  free(w->train_buf);
   free(w->xxF);
   free(w->xdF);
   free(w->yyF);
@@ -227,6 +230,7 @@ orc_ws *orc_ws_create(const mbt_config *c, int ibunch_mode, double *dist)
   const size_t per = (size_t)w->S * WS_NG * WS_NG;
   w->capF = (long)w->Nb * (long)per + 1;
   w->NriF = malloc(sizeof(double) * w->capF);
+  w->train_buf = malloc(sizeof(double) * WS_NC * ORC_ION_BLOCK * 2);
   w->xxF = malloc(sizeof(double) * w->capF);
   w->xdF = malloc(sizeof(double) * w->capF);
   w->yyF = malloc(sizeof(double) * w->capF);
@@ -537,26 +541,59 @@ static void fbi_train(orc_ws *w, ws_bunch *b, int ib, int kb, const double *Ex, 
     w->xxF[q] = w->xxF[q] + Dtime * w->xdF[q];
     w->yyF[q] = w->yyF[q] + Dtime * w->ydF[q];
   }
+  /* The reference loops slice-major (every ion of the cloud at slice 0, then at
+   * slice 1, ...).  An ion's path through the slices depends on nothing but its
+   * own state and the fields, so the ions are taken in blocks, every ion of a
+   * block carried through all slices on the threads at hand, with each
+   * Nri * E product kept; the per-slice sums then add those products in ion
+   * order, slice by slice -- the reference's own operations in its order, so
+   * the result is bitwise the same on any thread count. */
+  double sx[WS_NC], sy[WS_NC];
+  for (int cell = 0; cell < w->S; cell++) sx[cell] = sy[cell] = 0.0;
+  double *pe = w->train_buf;
+  for (long q0 = 0; q0 < w->nF; q0 += ORC_ION_BLOCK) {
+    const long nq = w->nF - q0 < ORC_ION_BLOCK ? w->nF - q0 : ORC_ION_BLOCK;
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < nq; j++) {
+      const long q = q0 + j;
+      double xx = w->xxF[q], xd = w->xdF[q], yy = w->yyF[q], yd = w->ydF[q];
+      const double nri = w->NriF[q];
+      for (int cell = 0; cell < w->S; cell++) {
+        double E[2];
+        interp(w, Ex, Ey, cell, xx, yy, E);
+        double ax = w->factF * E[0];
+        xd = xd + w->deltT * ax;
+        xx = xx + w->deltT * xd;
+        double ay = w->factF * E[1];
+        yd = yd + w->deltT * ay;
+        yy = yy + w->deltT * yd;
+        pe[((size_t)cell * ORC_ION_BLOCK + j) * 2] = nri * E[0];
+        pe[((size_t)cell * ORC_ION_BLOCK + j) * 2 + 1] = nri * E[1];
+      }
+      w->xxF[q] = xx;
+      w->xdF[q] = xd;
+      w->yyF[q] = yy;
+      w->ydF[q] = yd;
+    }
+#pragma omp parallel for schedule(static)
+    for (int cell = 0; cell < w->S; cell++) {
+      const double *p = pe + (size_t)cell * ORC_ION_BLOCK * 2;
+      double a = sx[cell], bb = sy[cell];
+      for (long j = 0; j < nq; j++) {
+        a = a + p[2 * j];
+        bb = bb + p[2 * j + 1];
+      }
+      sx[cell] = a;
+      sy[cell] = bb;
+    }
+  }
   for (int cell = 0; cell < w->S; cell++) {
     double dQ = Nelec * w->npop[cell] * K_E;
     double factjp = dQ > 0.0 ? factjp0 / dQ : 0.0;
-    double sx = 0.0, sy = 0.0;
-    for (long q = 0; q < w->nF; q++) {
-      double E[2];
-      interp(w, Ex, Ey, cell, w->xxF[q], w->yyF[q], E);
-      double ax = w->factF * E[0];
-      w->xdF[q] = w->xdF[q] + w->deltT * ax;
-      w->xxF[q] = w->xxF[q] + w->deltT * w->xdF[q];
-      double ay = w->factF * E[1];
-      w->ydF[q] = w->ydF[q] + w->deltT * ay;
-      w->yyF[q] = w->yyF[q] + w->deltT * w->ydF[q];
-      sx = sx + w->NriF[q] * E[0];
-      sy = sy + w->NriF[q] * E[1];
-    }
-    sx = factjp * sx;
-    sy = factjp * sy;
-    b->xp[cell] = b->xp[cell] + K_C * w->deltT * sx;
-    b->zp[cell] = b->zp[cell] + K_C * w->deltT * sy;
+    double fx = factjp * sx[cell];
+    double fy = factjp * sy[cell];
+    b->xp[cell] = b->xp[cell] + K_C * w->deltT * fx;
+    b->zp[cell] = b->zp[cell] + K_C * w->deltT * fy;
   }
   for (int q = 0; q < w->Nions; q++) {
     w->NriF[w->nF] = Nri[q];
