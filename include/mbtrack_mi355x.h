@@ -202,7 +202,8 @@ typedef struct mbt_ring mbt_ring;
                                    resistive-wall GEMM of ensembles (where the FP64 build uses
                                    one) in FP32 MFMA with hi/lo operand splitting and FP64 sums
                                    of FP32 chunks; a lone ring keeps its per-bunch FP64 RW sum,
-                                   which is latency-bound; coordinates, phase advance and the
+                                   which is latency-bound, unless MBT_RING_RW_GEMM is set;
+                                   coordinates, phase advance and the
                                    LR / RF-feedback recurrence stay FP64 */
 #define MBT_RING_RW_OVERLAP 16  /* resistive-wall sum without the GEMM: always split it into the
                                    taps of earlier turns' rows, summed by the leading blocks of
@@ -210,6 +211,14 @@ typedef struct mbt_ring mbt_ring;
                                    summed after the reduction.  Without the flag only rings of
                                    at least 4,194,304 macroparticles split it; below that one
                                    fused pass after the reduction is faster */
+#define MBT_RING_RW_GEMM 32     /* the GEMM form of the resistive-wall sum (FP64 MFMA, or FP32
+                                   MFMA with hi/lo splitting under MBT_RING_MIXED) from one ring
+                                   on, where by default it needs 8 rings sharing one tap table;
+                                   overrides MBT_RING_RW_OVERLAP */
+#define MBT_RING_HOST_INIT_DIST 64 /* make the initial distributions on the host and upload them;
+                                   by default rings whose distribution is the reference's Gaussian
+                                   generator with pairwise different plane seeds get theirs from a
+                                   device kernel, bit for bit the same */
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);
@@ -248,6 +257,10 @@ int mbt_ring_rw_history(const mbt_ring *r, double *out);
 int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches);
 /* Bytes of HBM the turn map pass must move per turn (algorithmic). */
 double mbt_ring_bytes_per_turn(const mbt_ring *r);
+/* No counterpart in the reference (a measurement aid): how many turn graphs
+ * (one per block of turns, reused for later blocks whose first monitor turn
+ * sits at the same offset) the ring has captured and instantiated so far. */
+int mbt_ring_graph_builds(const mbt_ring *r, long *count);
 /* Per-kernel timing with HIP events around every launch group (0 = off). */
 int mbt_ring_set_timing(mbt_ring *r, int on);
 long mbt_ring_turn(const mbt_ring *r);

@@ -21,6 +21,8 @@ MBT_RING_NO_GRAPHS = 2
 MBT_RING_NO_RW_GEMM = 4
 MBT_RING_MIXED = 8
 MBT_RING_RW_OVERLAP = 16
+MBT_RING_RW_GEMM = 32
+MBT_RING_HOST_INIT_DIST = 64
 MBT_WS_BUNCH_CURRENTS = 1
 MBT_WS_SERIAL_TRAIN = 2
 MBT_WS_MIXED = 4
@@ -73,6 +75,7 @@ def lib():
         L.mbt_ring_autotune.argtypes = [vp, ctypes.POINTER(ip), dp]
         L.mbt_ring_stream_ceiling.argtypes = [vp, ctypes.c_int, dp]
         L.mbt_ring_rf_state.argtypes = [vp, ip, dp]
+        L.mbt_ring_graph_builds.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
         L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
@@ -94,10 +97,11 @@ def _check(rc):
         raise MbtError("mbtrack error %d: %s" % (rc, lib().mbt_last_error().decode()))
 
 
-def _flags(init_dist, graphs, rw_gemm=True, mixed=False, rw_overlap=False):
+def _flags(init_dist, graphs, rw_gemm=True, mixed=False, rw_overlap=False, force_gemm=False, host_init=False):
     return ((0 if init_dist else MBT_RING_NO_INIT_DIST) | (0 if graphs else MBT_RING_NO_GRAPHS)
             | (0 if rw_gemm else MBT_RING_NO_RW_GEMM) | (MBT_RING_MIXED if mixed else 0)
-            | (MBT_RING_RW_OVERLAP if rw_overlap else 0))
+            | (MBT_RING_RW_OVERLAP if rw_overlap else 0) | (MBT_RING_RW_GEMM if force_gemm else 0)
+            | (MBT_RING_HOST_INIT_DIST if host_init else 0))
 
 
 def dptr(a):
@@ -153,14 +157,16 @@ class Config:
 class Ring:
     """All bunches of one ring resident on one GPU (mbt_ring_* C ABI)."""
 
-    def __init__(self, cfg, device=0, init_dist=True, graphs=True, mixed=False, rw_overlap=False):
+    def __init__(self, cfg, device=0, init_dist=True, graphs=True, mixed=False, rw_overlap=False, force_gemm=False,
+                 host_init=False):
         L = lib()
         self.cfg = cfg
         self.Nb = cfg.geti("Nbunch")
         self.Np = cfg.geti("Np")
         self.Nc = cfg.geti("Ncell")
         p = ctypes.c_void_p()
-        _check(L.mbt_ring_create(cfg.ptr, device, _flags(init_dist, graphs, mixed=mixed, rw_overlap=rw_overlap),
+        _check(L.mbt_ring_create(cfg.ptr, device, _flags(init_dist, graphs, mixed=mixed, rw_overlap=rw_overlap,
+                                                         force_gemm=force_gemm, host_init=host_init),
                                  ctypes.byref(p)))
         self.ptr = p
 
@@ -234,6 +240,12 @@ class Ring:
         _check(lib().mbt_ring_autotune(self.ptr, ctypes.byref(best), dptr(ms)))
         return best.value, {k: v for k, v in zip((1, 2, 4, 8, 16, 32, 64), ms.tolist()) if v > 0}
 
+    def graph_builds(self):
+        """Turn graphs captured and instantiated so far."""
+        n = ctypes.c_long()
+        _check(lib().mbt_ring_graph_builds(self.ptr, ctypes.byref(n)))
+        return n.value
+
     def stream_ceiling(self, reps=10):
         """GB/s of a plain in-place update of the ring's own coordinate arrays
         (bit-preserving): the ceiling their placement sets for the turn pass."""
@@ -248,7 +260,8 @@ class Ensemble(Ring):
     Bunch ordinals, stats and ampinv rows run over the rings' bunches in ring
     order; rf rows hold one (phai0, Vrf0) pair per ring."""
 
-    def __init__(self, cfgs, device=0, init_dist=True, graphs=True, rw_gemm=True, mixed=False, rw_overlap=False):
+    def __init__(self, cfgs, device=0, init_dist=True, graphs=True, rw_gemm=True, mixed=False, rw_overlap=False,
+                 host_init=False):
         L = lib()
         self.cfgs = list(cfgs)
         self.R = len(self.cfgs)
@@ -260,7 +273,8 @@ class Ensemble(Ring):
         self.first = np.concatenate([[0], np.cumsum(self.nb)])
         arr = (ctypes.c_void_p * self.R)(*[c.ptr.value for c in self.cfgs])
         p = ctypes.c_void_p()
-        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs, rw_gemm, mixed, rw_overlap),
+        _check(L.mbt_ensemble_create(arr, self.R, device, _flags(init_dist, graphs, rw_gemm, mixed, rw_overlap,
+                                                                 host_init=host_init),
                                      ctypes.byref(p)))
         self.ptr = p
 

@@ -1722,8 +1722,8 @@ struct RingHost {
 #define NGRAPH 4
 struct GraphEntry {
   hipGraphExec_t exec;
-  long long phase; // first turn modulo NrevMon
-  long n;          // turns
+  long mon; // offset of the block's first monitor turn, -1 when it has none
+  long n;   // turns
 };
 
 struct mbt_ring {
@@ -1736,6 +1736,7 @@ struct mbt_ring {
   long long Ntot;   // macroparticles
   long long Nalloc; // coordinate slots: bunches padded to BUNCH_ALIGN
   int rwg_on;        // RW sum as one GEMM over the ensemble's RW rings
+  int force_gemm;    // MBT_RING_RW_GEMM: the GEMM form from one RW ring on
   int mixed;         // MBT_RING_MIXED: FP32 wake convolutions, FP32 hi/lo RW GEMM
   RWGemm rwg;
   int *rwg_col_ring, *rwg_bucket_bunch;
@@ -1767,6 +1768,7 @@ struct mbt_ring {
   int rw_split;        // RW sum split: earlier rows' taps in k_turn's leading blocks, then k_rw_fresh
   GraphEntry graphs[NGRAPH];
   int graph_next, use_graphs;
+  long graph_builds; // turn graphs captured and instantiated so far
   LRConst *lc;
   int *lr_first; // [R+1] first resonator of each ring in lc
   int nres;
@@ -2074,7 +2076,7 @@ static int setup_rw_gemm(mbt_ring *r)
       return MBT_OK; // rings with different taps: the per-bunch sum
     nrw++;
   }
-  if (nrw < RWG_MIN_RINGS) return MBT_OK;
+  if (nrw < (r->force_gemm ? 1 : RWG_MIN_RINGS)) return MBT_OK;
   RWGemm &g = r->rwg;
   g.h = h;
   g.M = (Nmlt + 1) * h;
@@ -2165,6 +2167,107 @@ static int setup_rw_split(mbt_ring *r)
   return MBT_OK;
 }
 
+// ---- initial distributions on the device -------------------------------------
+// The reference's Gaussian bunch generator (weak_generate_bunch_distribution,
+// src/bunch_weak.c:85-105, with c_fnorm, src/bunch.c:379-405) restated for the
+// device, bit for bit: per plane, glibc's TYPE_3 rand() stream seeded with
+// iseed[plane] + bucket, each normal the sum of 12 uniforms rand() / RAND_MAX
+// minus 1/2, the position ps * n1 + offset and the slope so + ss * n2 + cr * ps * n1.
+// The reference runs one MPI rank per bunch whose generator carries over from
+// plane to plane; it reseeds at each plane whenever the planes' seeds differ,
+// so with pairwise different seeds every (bunch, plane) stream starts fresh and
+// one lane generates it.  The 31-word additive-feedback table of each lane lives
+// in LDS (an odd stride of words per lane, so lanes reading the same index hit
+// different banks).
+struct GenRing {
+  int on;       // the device generator covers this ring
+  int track[3]; // planes generated (LON always)
+  int iseed[3];
+  double po[3], ps[3], so[3], ss[3], cr[3];
+};
+
+#define GEN_WG 64
+#define GEN_TAB 33
+
+__device__ __forceinline__ int gen_next(int *t, int &f, int &b)
+{
+  const unsigned v = (unsigned)t[f] + (unsigned)t[b];
+  t[f] = (int)v;
+  f = f == 30 ? 0 : f + 1;
+  b = b == 30 ? 0 : b + 1;
+  return (int)(v >> 1);
+}
+
+// srandom_r for TYPE_3: Park-Miller minimal standard words (Schrage's
+// decomposition, signed 32-bit word), front index 3 words ahead of the rear,
+// the first 310 outputs discarded
+__device__ void gen_srand(int *t, int &f, int &b, unsigned seed)
+{
+  if (seed == 0) seed = 1;
+  int word = (int)seed;
+  t[0] = word;
+  for (int i = 1; i < 31; i++) {
+    const long long hi = word / 127773, lo = word % 127773;
+    long long w = 16807 * lo - 2836 * hi;
+    if (w < 0) w += 2147483647;
+    word = (int)w;
+    t[i] = word;
+  }
+  f = 3;
+  b = 0;
+  for (int k = 0; k < 310; k++) (void)gen_next(t, f, b);
+}
+
+__device__ __forceinline__ double gen_fnorm(int *t, int &f, int &b)
+{
+  double sum = 0.0;
+  for (int i = 0; i < 12; i++) {
+    const double u = ((double)gen_next(t, f, b)) / ((double)2147483647);
+    const double v = u - 0.5;
+    sum = sum + v;
+  }
+  return sum;
+}
+
+// one lane per (bunch, plane)
+__global__ __launch_bounds__(GEN_WG) void k_gen(const GenRing *gr, const int *bunch_ring, const int *bucket,
+                                                const long long *bunch_first, const int *np, Coords C, int G)
+{
+  __shared__ int tabs[GEN_WG * GEN_TAB];
+  const int i = blockIdx.x * GEN_WG + threadIdx.x;
+  if (i >= 3 * G) return;
+  const int g = i / 3, p = i % 3;
+  const GenRing &R = gr[bunch_ring[g]];
+  if (!R.on || !R.track[p]) return;
+  int *t = tabs + threadIdx.x * GEN_TAB;
+  int f, b;
+  gen_srand(t, f, b, (unsigned)(R.iseed[p] + bucket[g]));
+  double *pos = p == 0 ? C.tau : p == 1 ? C.x : C.z;
+  double *slo = p == 0 ? C.del : p == 1 ? C.xp : C.zp;
+  const long long first = bunch_first[g];
+  const int Np = np[g];
+  const double po = R.po[p], ps = R.ps[p], so = R.so[p], ss = R.ss[p], cr = R.cr[p];
+  for (int j = 0; j < Np; j++) {
+    const double tp = ps * gen_fnorm(t, f, b);
+    pos[first + j] = po + tp;
+    const double n2 = gen_fnorm(t, f, b);
+    slo[first + j] = so + ss * n2 + cr * tp;
+  }
+}
+
+// Whether the device generator reproduces the ring's initial distribution:
+// Gaussian planes only (no distribution files, no head-tail mode excitation),
+// more than one macroparticle, pairwise different seeds of the planes it makes.
+static int gen_on_device(const mbt_config *c)
+{
+  if (c->Np <= 1 || !c->nGen[0]) return 0;
+  for (int p = 1; p < 3; p++)
+    if (c->TrackPlane[p] && (!c->nGen[p] || (p == 2 && c->mode_excitation))) return 0;
+  if (c->TrackPlane[1] && c->iseed[1] == c->iseed[0]) return 0;
+  if (c->TrackPlane[2] && (c->iseed[2] == c->iseed[0] || (c->TrackPlane[1] && c->iseed[2] == c->iseed[1]))) return 0;
+  return 1;
+}
+
 static int create_rings(const mbt_config *const *cfgs, int R, int device, int flags, mbt_ring **out)
 {
   *out = NULL;
@@ -2191,6 +2294,7 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   r->rh = (RingHost *)calloc(R, sizeof(RingHost));
   r->Ph = (DevParams *)calloc(R, sizeof(DevParams));
   r->mixed = (flags & MBT_RING_MIXED) ? 1 : 0;
+  r->force_gemm = (flags & MBT_RING_RW_GEMM) ? 1 : 0;
   for (int k = 0; k < R; k++) copy_cfg(&r->rh[k], cfgs[k]);
   CK(hipSetDevice(device));
   // a blocking stream: ordered after work on the null stream (allocation-time
@@ -2509,9 +2613,47 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
     mbt_ring_destroy(r);
     return MBT_EINVAL;
   }
-  // ---- initial distributions (host threads, one glibc stream per bunch)
+  // ---- initial distributions: on the device where its generator covers the
+  // ring, otherwise host threads (one glibc stream per bunch) and an upload
   if (!(flags & MBT_RING_NO_INIT_DIST)) {
+    GenRing *gh = (GenRing *)calloc(R, sizeof(GenRing));
+    int ndev = 0;
     for (int k = 0; k < R; k++) {
+      const mbt_config *cf = &r->rh[k].cfg;
+      GenRing &q = gh[k];
+      q.on = !(flags & MBT_RING_HOST_INIT_DIST) && gen_on_device(cf);
+      ndev += q.on;
+      for (int p = 0; p < 3; p++) {
+        q.track[p] = p == 0 ? 1 : cf->TrackPlane[p] != 0;
+        q.iseed[p] = cf->iseed[p];
+        q.po[p] = cf->pos_offset[p];
+        q.ps[p] = cf->pos_sgm[p];
+        q.so[p] = cf->slope_offset[p];
+        q.ss[p] = cf->slope_sgm[p];
+        q.cr[p] = cf->correlate[p];
+      }
+    }
+    if (ndev > 0) {
+      GenRing *gd = NULL;
+      int rcg = dalloc(&gd, (size_t)R);
+      if (!rcg && hipMemcpy(gd, gh, sizeof(GenRing) * R, hipMemcpyHostToDevice) != hipSuccess) rcg = MBT_EDEVICE;
+      if (!rcg) {
+        hipLaunchKernelGGL(k_gen, dim3((unsigned)((3LL * G + GEN_WG - 1) / GEN_WG)), dim3(GEN_WG), 0, r->stream, gd,
+                           r->bunch_ring, r->bucket, r->bunch_first, r->np, r->C, G);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(r->stream) != hipSuccess) {
+          mbt_set_error("initial distribution kernel failed");
+          rcg = MBT_EDEVICE;
+        }
+      }
+      if (gd) (void)hipFree(gd);
+      if (rcg) {
+        free(gh);
+        mbt_ring_destroy(r);
+        return rcg;
+      }
+    }
+    for (int k = 0; k < R; k++) {
+      if (gh[k].on) continue;
       const mbt_config *cf = &r->rh[k].cfg;
       const size_t per = (size_t)6 * cf->Np;
       double *buf = (double *)malloc(sizeof(double) * per * cf->Nbunch);
@@ -2524,10 +2666,12 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
         rc = mbt_ensemble_set_bunch(r, r->Ph[k].bunch_off + b, buf + per * b);
       free(buf);
       if (rc) {
+        free(gh);
         mbt_ring_destroy(r);
         return rc;
       }
     }
+    free(gh);
   }
   if (hipDeviceSynchronize() != hipSuccess) {
     mbt_set_error("device synchronisation after set-up failed");
@@ -2817,14 +2961,16 @@ static int enqueue_turn(mbt_ring *r, hipEvent_t *E)
 
 // n turns from the current turn as one HIP graph.  The launches of a turn
 // depend on the turn number only through the monitor-turn variant of k_turn,
-// so a graph is reusable for any block of n turns at the same phase modulo
-// NrevMon; a few are cached.  (Timed runs launch eagerly: the runtime does not
-// time events recorded inside a graph.)
+// and the monitor turns of a block follow from its first one, so a graph is
+// reusable for any block of n turns whose first monitor turn sits at the same
+// offset (or that has none); a few are cached.  (Timed runs launch eagerly:
+// the runtime does not time events recorded inside a graph.)
 static int turn_graph(mbt_ring *r, long n, hipGraphExec_t *out)
 {
-  const long long phase = r->rev_host % r->nrev_mon;
+  long mon = (long)(r->nrev_mon - 1 - r->rev_host % r->nrev_mon);
+  if (mon >= n) mon = -1;
   for (int i = 0; i < NGRAPH; i++)
-    if (r->graphs[i].exec && r->graphs[i].phase == phase && r->graphs[i].n == n) {
+    if (r->graphs[i].exec && r->graphs[i].mon == mon && r->graphs[i].n == n) {
       *out = r->graphs[i].exec;
       return MBT_OK;
     }
@@ -2856,9 +3002,20 @@ static int turn_graph(mbt_ring *r, long n, hipGraphExec_t *out)
     mbt_set_error("turn graph instantiation failed: %s", hipGetErrorString(e));
     return MBT_EDEVICE;
   }
-  g.phase = phase;
+  g.mon = mon;
   g.n = n;
+  r->graph_builds++;
   *out = g.exec;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ring_graph_builds(const mbt_ring *r, long *count)
+{
+  if (!r || !count) {
+    mbt_set_error("mbt_ring_graph_builds: null argument");
+    return MBT_EINVAL;
+  }
+  *count = r->graph_builds;
   return MBT_OK;
 }
 

@@ -182,13 +182,14 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 }
 
 // get_linearly_interpolated_Efields, src/fbi_weakstrong.c:454-580, on one
-// slice's 25 field values.  The grid quotients multiply by the reciprocal
-// spacings (iax = 1 / aAx, iaz = 1 / aAz) in place of the reference's four FP64
-// divisions per interaction; such a quotient can differ from the divided one in
-// its last bit, which moves a cell choice only where the bilinear field is
-// continuous.
-__device__ __forceinline__ void ws_interp(const double *ex, const double *ey, double aAx, double aAz, double iax,
-                                          double iaz, double xx, double yy, double &Ex, double &Ey)
+// slice's 25 field values, stored as (Ex, Ey) pairs so that one 16-byte LDS
+// read fetches both fields of a grid point.  The grid quotients multiply by the
+// reciprocal spacings (iax = 1 / aAx, iaz = 1 / aAz) in place of the
+// reference's four FP64 divisions per interaction; such a quotient can differ
+// from the divided one in its last bit, which moves a cell choice only where
+// the bilinear field is continuous.
+__device__ __forceinline__ void ws_interp(const double2 *t, double aAx, double aAz, double iax, double iaz, double xx,
+                                          double yy, double &Ex, double &Ey)
 {
   // The reference's two sign cases are mirror images: a non-negative coordinate
   // interpolates from cell k1 = trunc(x / d) towards +2, a negative one from k1
@@ -196,17 +197,20 @@ __device__ __forceinline__ void ws_interp(const double *ex, const double *ey, do
   // sign-symmetric, as is every rounding below) and mirrored: with
   // a = min(trunc(|x| / d), 2), k1 = s a and k2 = s min(a + 1, 2) for the sign
   // s, and the signed fraction s (x - k1 d) / d equals (|x| - a d) / d exactly.
+  // Grid point (kx, kz), kx, kz in [-2, 2], is entry 12 + 5 kx + kz; the
+  // partner is one step further out in each plane (none at the clamped edge).
   const int ax = min((int)(fabs(xx) * iax), 2), az = min((int)(fabs(yy) * iaz), 2);
-  const int bx = min(ax + 1, 2), bz = min(az + 1, 2);
   const bool px = xx >= 0, pz = yy >= 0;
-  const int x1 = px ? 2 + ax : 2 - ax, x2 = px ? 2 + bx : 2 - bx;
-  const int z1 = pz ? 2 + az : 2 - az, z2 = pz ? 2 + bz : 2 - bz;
-  const int i1 = x1 * 5 + z1, i2 = x2 * 5 + z1, i3 = x1 * 5 + z2, i4 = x2 * 5 + z2;
+  const int ox = 5 * ax, sx5 = ax < 2 ? 5 : 0, sz1 = az < 2 ? 1 : 0;
+  const int i1 = 12 + (px ? ox : -ox) + (pz ? az : -az);
+  const int i2 = i1 + (px ? sx5 : -sx5);
+  const int dz = pz ? sz1 : -sz1;
+  const double2 e1 = t[i1], e2 = t[i2], e3 = t[i1 + dz], e4 = t[i2 + dz];
   const double sx = (fabs(xx) - ax * aAx) * iax, sz = (fabs(yy) - az * aAz) * iaz;
-  const double y5 = ey[i1] + sx * (ey[i2] - ey[i1]);
-  const double y6 = ey[i3] + sx * (ey[i4] - ey[i3]);
-  const double x5 = ex[i1] + sx * (ex[i2] - ex[i1]);
-  const double x6 = ex[i3] + sx * (ex[i4] - ex[i3]);
+  const double y5 = e1.y + sx * (e2.y - e1.y);
+  const double y6 = e3.y + sx * (e4.y - e3.y);
+  const double x5 = e1.x + sx * (e2.x - e1.x);
+  const double x6 = e3.x + sx * (e4.x - e3.x);
   Ey = y5 + sz * (y6 - y5);
   Ex = x5 + sz * (x6 - x5);
 }
@@ -215,7 +219,7 @@ __device__ __forceinline__ void ws_interp(const double *ex, const double *ey, do
 
 __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev, int slot, int monitor)
 {
-  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  __shared__ double2 exy_s[WS_PTS];
   const int b = blockIdx.x, tid = threadIdx.x, S = P.S;
   const int kb = D.bucket[b];
   double *xtau = D.sl + (size_t)b * 6 * S, *xeps = xtau + S, *xx = xeps + S, *xp = xx + S, *zz = xp + S,
@@ -307,8 +311,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev
     }
     const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
     const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
-    ex_s[q] = Ex;
-    ey_s[q] = Ey;
+    exy_s[q] = make_double2(Ex, Ey);
     D.Ex[(size_t)b * npts + q] = Ex;
     D.Ey[(size_t)b * npts + q] = Ey;
   }
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev
     double x = xx[cell] + msx * P.aLx, xd = 0.0, y = zz[cell] + msz * P.aLz, yd = 0.0;
     for (int c = cell + 1; c < S; c++) {
       double Ex, Ey;
-      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+      ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
       const double ax = P.factF * Ex;
       xd = xd + P.deltT * ax;
       x = x + P.deltT * xd;
@@ -439,15 +442,12 @@ __global__ __launch_bounds__(WS_WG) void k_ws_cloud_reset(WSP P, WSD D, int slot
 // bunch's ions
 __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long long nF, double Dtime, int slot)
 {
-  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  __shared__ double2 exy_s[WS_PTS];
   __shared__ double red[WS_WG / 64][2];
   __shared__ int last;
   const int tid = threadIdx.x, S = P.S, npts = S * 25;
   const size_t GP = (size_t)P.G * npts;
-  for (int q = tid; q < npts; q += WS_WG) {
-    ex_s[q] = D.Ex[(size_t)ib * npts + q];
-    ey_s[q] = D.Ey[(size_t)ib * npts + q];
-  }
+  for (int q = tid; q < npts; q += WS_WG) exy_s[q] = make_double2(D.Ex[(size_t)ib * npts + q], D.Ey[(size_t)ib * npts + q]);
   __syncthreads();
   const long long q = (long long)blockIdx.x * WS_WG + tid;
   const bool act = q < nF;
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
     double sx = 0.0, sy = 0.0;
     if (act) {
       double Ex, Ey;
-      ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+      ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
       const double ax = P.factF * Ex;
       xd = xd + P.deltT * ax;
       x = x + P.deltT * xd;
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
 // each the fixed-order sum of its four waves' sums.
 __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
 {
-  __shared__ double ex_s[WS_PTS], ey_s[WS_PTS];
+  __shared__ double2 exy_s[WS_PTS];
   __shared__ double wred[2 * WS_S + 3][WS_WG / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25, NQ = 2 * S + 3;
   const size_t GP = (size_t)P.G * npts;
@@ -591,10 +591,8 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
   }
   for (int ib = jmin + 1; ib < P.G; ib++) {
     __syncthreads(); // the previous bunch's table and sums have been read
-    for (int k = tid; k < npts; k += WS_WG) {
-      ex_s[k] = D.Ex[(size_t)ib * npts + k];
-      ey_s[k] = D.Ey[(size_t)ib * npts + k];
-    }
+    for (int k = tid; k < npts; k += WS_WG)
+      exy_s[k] = make_double2(D.Ex[(size_t)ib * npts + k], D.Ey[(size_t)ib * npts + k]);
     __syncthreads();
     const bool act = j < ib;
     if (act) {
@@ -611,7 +609,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
         double sx = 0.0, sy = 0.0;
         if (act && c < S) {
           double Ex, Ey;
-          ws_interp(ex_s + c * 25, ey_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+          ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
           const double ax = P.factF * Ex;
           xd = xd + P.deltT * ax;
           x = x + P.deltT * xd;

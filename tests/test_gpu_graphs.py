@@ -59,3 +59,39 @@ def test_timed_graphs(tmp_path):
         assert (ms > 0).all()
     a.close()
     b.close()
+
+
+def test_graph_cache_keyed_by_monitor_offset(tmp_path):
+    """A block of turns reuses a cached graph whenever its first monitor turn
+    sits at the same offset, or when neither block holds a monitor turn: the
+    pattern a benchmark uses (an untimed block, then timed blocks of the same
+    length at other phases) instantiates no graph inside the timed blocks."""
+    path = write_variant("wakes", str(tmp_path), tracking={"NrevMon": 100, "NrevTot": 200})
+    cfg = m.Config(path, str(tmp_path) + "/")
+    a = m.Ring(cfg, graphs=True)
+    b = m.Ring(cfg, graphs=False)
+    for r in (a, b):
+        r.prepare()
+    a.track(5)
+    b.track(5)
+    assert a.graph_builds() == 1
+    a.track(20)  # turns 5..24: no monitor turn
+    b.track(20)
+    assert a.graph_builds() == 2
+    sa, _, _ = a.track(20)  # turns 25..44: the same graph
+    sb, _, _ = b.track(20)
+    assert a.graph_builds() == 2
+    assert np.array_equal(sa, sb)
+    a.track(20)  # turns 45..64
+    b.track(20)
+    assert a.graph_builds() == 2
+    sa, aa, _ = a.track(20)  # turns 65..84
+    sb, ab, _ = b.track(20)
+    assert a.graph_builds() == 2
+    sa, aa, _ = a.track(20)  # turns 85..104: monitor turn 99 at offset 14, a new graph
+    sb, ab, _ = b.track(20)
+    assert a.graph_builds() == 3
+    assert np.array_equal(sa, sb) and np.array_equal(aa, ab)
+    assert np.any(aa[14] != 0) and not np.any(aa[13] != 0)
+    a.close()
+    b.close()
