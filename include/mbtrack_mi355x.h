@@ -358,6 +358,13 @@ int mbt_manager_weakweak(const mbt_config *cfg, int device);
 int mbt_wofz(int device, long n, const double *x, const double *y, double *wr, double *wi, int *flag);
 /* The same in FP32 (the mixed-precision beam-ion field); no reference counterpart. */
 int mbt_wofz_f32(int device, long n, const float *x, const float *y, float *wr, float *wi, int *flag);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mbt_wofz with its device staging in a fixed array of the
+ * library (9 MiB per device, held until the process ends) instead of hipMalloc'ed
+ * buffers; at most 262,144 points per call (more: MBT_EINVAL, nothing computed).
+ * Calls that overlap on one device share that array and race. */
+int mbt_wofz_noalloc(int device, long n, const double *x, const double *y, double *wr, double *wi, int *flag);
 
 #ifdef __cplusplus
 }

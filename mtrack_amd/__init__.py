@@ -16,6 +16,7 @@ EXE = os.path.join(HERE, "mbtrack-mi355x")
 
 NSTAT = 12
 NAMP = 6
+MBT_EINVAL = -1
 MBT_RING_NO_INIT_DIST = 1
 MBT_RING_NO_GRAPHS = 2
 MBT_RING_NO_RW_GEMM = 4
@@ -88,6 +89,8 @@ def lib():
             L.mbt_manager_weakweak.argtypes = [vp, ip]
         if hasattr(L, "mbt_wofz"):
             L.mbt_wofz.argtypes = [ip, ctypes.c_long, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        if hasattr(L, "mbt_wofz_noalloc"):
+            L.mbt_wofz_noalloc.argtypes = [ip, ctypes.c_long, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
