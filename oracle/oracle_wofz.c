@@ -17,6 +17,7 @@
  */
 #define _GNU_SOURCE /* sincos */
 #include <math.h>
+#include "oracle_sens.h"
 #include "oracle.h"
 
 /* separate calls that the compiler cannot pair into sincos */
@@ -48,8 +49,10 @@ int orc_wofz(double xi, double yi, double *u, double *v)
     }
     double a = -two_over_sqrtpi * (sr * ay + si * ax) + 1.0;
     double b = two_over_sqrtpi * (sr * ax - si * ay);
-    double e = exp(-re2), sn, cs;
+    double e = sens_ulp_exp(-re2), sn, cs;
     sincos(im2, &sn, &cs);
+    sn = sens_ulp(sn, im2);
+    cs = sens_ulp(cs, -im2);
     er = e * cs;
     ei = -e * sn;
     wr = a * er - b * ei;
@@ -93,7 +96,7 @@ int orc_wofz(double xi, double yi, double *u, double *v)
       wr = two_over_sqrtpi * sr;
       wi = two_over_sqrtpi * si;
     }
-    if (ay == 0.0) wr = exp(-ax * ax);
+    if (ay == 0.0) wr = sens_ulp_exp(-ax * ax);
   }
   if (yi < 0.0) {
     if (series) {
@@ -102,9 +105,9 @@ int orc_wofz(double xi, double yi, double *u, double *v)
     } else {
       double mre2 = -re2;
       if (im2 > big_trig || mre2 > big_exp) return 1;
-      double e = 2.0 * exp(mre2);
-      er = e * lib_cos(im2);
-      ei = -e * lib_sin(im2);
+      double e = 2.0 * sens_ulp_exp(mre2);
+      er = e * sens_ulp(lib_cos(im2), im2);
+      ei = -e * sens_ulp(lib_sin(im2), -im2);
     }
     wr = er - wr;
     wi = ei - wi;

@@ -36,6 +36,7 @@
 #include <string.h>
 
 #include "oracle.h"
+#include "oracle_sens.h"
 
 #define WS_NC 50        /* slice arrays of bunch_strong_t, src/bunch.h:113-121 */
 #define WS_OFF 2        /* grid offset (Ngrid 5), src/bunch.c:820-827 */
@@ -294,7 +295,7 @@ static void shortrange(const orc_ws *w, ws_bunch *b)
 {
   const mbt_config *c = w->c;
   for (int ic = 0; ic < w->S; ic++) {
-    double gain = c->Vrf0 / (c->E0 * 1.0e+03) * sin(c->wrf * b->xtau[ic] + c->phai0);
+    double gain = c->Vrf0 / (c->E0 * 1.0e+03) * sens_ulp(sin(c->wrf * b->xtau[ic] + c->phai0), b->xtau[ic]);
     double e0 = b->xeps[ic];
     b->xeps[ic] = e0 + gain - c->Urad;
     b->xtau[ic] = b->xtau[ic] - e0 * c->T0 * c->ac;
@@ -309,7 +310,12 @@ static void interp(const orc_ws *w, const double *Ex, const double *Ey, int ic, 
 {
   const int off = WS_OFF;
   const double ax = w->agABSx, az = w->agABSz;
+  #ifdef ORC_SENS_RECIP
+  const double iax = 1.0 / ax, iaz = 1.0 / az;
+  int k1x = (int)(xx * iax), k1z = (int)(yy * iaz), k2x, k2z;
+#else
   int k1x = (int)(xx / ax), k1z = (int)(yy / az), k2x, k2z;
+#endif
   const double *ex = Ex + ic * WS_NG * WS_NG, *ey = Ey + ic * WS_NG * WS_NG;
 #define IDX(kx, kz) (((kx) + off) * WS_NG + ((kz) + off))
   double sx, sz; /* +1 / -1: direction of the interpolation */
@@ -337,7 +343,11 @@ static void interp(const orc_ws *w, const double *Ex, const double *Ey, int ic, 
   }
   const int i1 = IDX(k1x, k1z), i2 = IDX(k2x, k1z), i3 = IDX(k1x, k2z), i4 = IDX(k2x, k2z);
 #undef IDX
+  #ifdef ORC_SENS_RECIP
+  const double fx = (xx - k1x * ax) * iax, fz = (yy - k1z * az) * iaz;
+#else
   const double fx = (xx - k1x * ax) / ax, fz = (yy - k1z * az) / az;
+#endif
   double y5, y6, x5, x6;
   if (sx > 0) {
     y5 = ey[i1] + fx * (ey[i2] - ey[i1]);
@@ -380,7 +390,7 @@ static void fbi_bunch(orc_ws *w, const ws_bunch *b, int kb)
         double yi1 = Dy / fBE, yi2 = yi1 * sxx / szz;
         double u1[2], u2[2];
         if (orc_wofz(xi1, yi1, &u1[0], &u1[1]) || orc_wofz(xi2, yi2, &u2[0], &u2[1])) w->wofz_fail++;
-        double g = exp(-(Dx * Dx * 1.0e+06 / (2 * sxx * sxx) + Dy * Dy * 1.0e+06 / (2 * szz * szz)));
+        double g = sens_ulp_exp(-(Dx * Dx * 1.0e+06 / (2 * sxx * sxx) + Dy * Dy * 1.0e+06 / (2 * szz * szz)));
         w->Ex[ic] = -factdQ * (u1[1] - g * u2[1]);
         w->Ey[ic] = -factdQ * (u1[0] - g * u2[0]);
         ic++;
@@ -578,6 +588,10 @@ static void fbi_train(orc_ws *w, ws_bunch *b, int ib, int kb, const double *Ex, 
 #pragma omp parallel for schedule(static)
     for (int cell = 0; cell < w->S; cell++) {
       const double *p = pe + (size_t)cell * ORC_ION_BLOCK * 2;
+#ifdef ORC_SENS_TREE
+      sx[cell] = sx[cell] + sens_pairwise(p, nq, 2);
+      sy[cell] = sy[cell] + sens_pairwise(p + 1, nq, 2);
+#else
       double a = sx[cell], bb = sy[cell];
       for (long j = 0; j < nq; j++) {
         a = a + p[2 * j];
@@ -585,6 +599,7 @@ static void fbi_train(orc_ws *w, ws_bunch *b, int ib, int kb, const double *Ex, 
       }
       sx[cell] = a;
       sy[cell] = bb;
+#endif
     }
   }
   for (int cell = 0; cell < w->S; cell++) {
@@ -689,11 +704,15 @@ int orc_ws_track(orc_ws *w, long nturns, double *cm, double *mon, double *ion, d
       }
       if (ion) {
         double sx = 0.0, sy = 0.0, sn = 0.0;
+#ifdef ORC_SENS_TREE
+        sens_cloud_sums(w->NriF, w->xxF, w->yyF, w->nF, &sx, &sy, &sn);
+#else
         for (long q = 0; q < w->nF; q++) {
           sx = sx + w->NriF[q] * w->xxF[q];
           sy = sy + w->NriF[q] * w->yyF[q];
           sn = sn + w->NriF[q];
         }
+#endif
         double *o = ion + ((size_t)t * w->Nb + ib) * 3;
         o[0] = sx;
         o[1] = sy;

@@ -38,6 +38,7 @@
 #include <string.h>
 #include <math.h>
 #include <time.h>
+#include <omp.h>
 #include "types.h"
 #include "bunch.h"
 #include "input.h"
@@ -84,6 +85,8 @@ int main(int argc, char **argv)
   long nturns_override = argc > 3 ? atol(argv[3]) : -1;
   int dump_hist = argc > 4 ? atoi(argv[4]) : 0;
   int dump_parts_kb = argc > 5 ? atoi(argv[5]) : -1;
+  /* timing: the first serial_turns turns on one thread, the rest on all */
+  long serial_turns = argc > 6 ? atol(argv[6]) : 0;
   char conf[FILENAME_MAX];
   snprintf(conf, sizeof conf, "%s", argv[1]);
   snprintf(track.work_path, FILENAME_MAX, "%s", argv[2]);
@@ -296,7 +299,18 @@ int main(int argc, char **argv)
   int *Nstat = calloc(Nb, sizeof(int));
   struct timespec t_a, t_b;
   clock_gettime(CLOCK_MONOTONIC, &t_a);
+  /* The reference runs one MPI rank per bunch; here the per-bunch work of a turn
+     runs on the host's cores (OpenMP threads over bunches, each bunch's own
+     state only) wherever the reference's code touches no shared state: not with
+     quantum excitation, whose glibc rand stream and Box-Muller cache are
+     process-wide.  Files are written after each parallel loop, in bunch order. */
+  const int qe_off = !track.EnableQuantum;
+  struct timespec t_s;
+  t_s = t_a;
   for (long rev = 0; rev < track.NrevTot; rev++) {
+    const int par = qe_off && rev >= serial_turns;
+    if (rev == serial_turns && serial_turns > 0) clock_gettime(CLOCK_MONOTONIC, &t_s);
+#pragma omp parallel for schedule(dynamic, 1) if (par)
     for (int k = 0; k < Nb; k++) {
       worker_t *w = &W[k];
       if (w->sf.PlaneL + w->sf.PlaneV + w->sf.PlaneH + lr_tot > 0)
@@ -314,6 +328,9 @@ int main(int argc, char **argv)
       int iseed = mp.iseed[LON] + w->bunch.kb;
       transform_weak_bunch_optic(&w->bunch, &mp, iseed + rev, w->kb, &w->ring);
       weak_bunch_calc_statistics(&w->bunch, &track);
+    }
+    for (int k = 0; k < Nb; k++) {
+      worker_t *w = &W[k];
       dump_stats(fst, rev, w->kb, &w->bunch.stats);
       if (w->bunch.kb_out == 1) {
         Nstat[k]++;
@@ -331,15 +348,10 @@ int main(int argc, char **argv)
         mom[0] = W[k].bunch.stats.pos.x;
         mom[1] = W[k].bunch.stats.pos.z;
       }
+#pragma omp parallel for schedule(dynamic, 1) if (par)
     for (int k = 0; k < Nb; k++) {
       worker_t *w = &W[k];
-      if ((rev + 1) % track.NrevMon == 0) {
-        weak_bunch_calc_ampinv(&w->bunch, &track);
-        const bunch_stats_t *s = &w->bunch.stats;
-        fprintf(famp, "%ld %d %.17g %.17g %.17g %.17g %.17g %.17g\n", rev, w->kb,
-                s->ampinv.v[0], s->ampinv.v[1], s->ampinv.v[2],
-                s->ampinv_cm.v[0], s->ampinv_cm.v[1], s->ampinv_cm.v[2]);
-      }
+      if ((rev + 1) % track.NrevMon == 0) weak_bunch_calc_ampinv(&w->bunch, &track);
       if (track.EnableRW_long) {
         const int bpos = -rev * h + w->kb;
         if (track.TrackPlane[HOR])
@@ -349,6 +361,14 @@ int main(int argc, char **argv)
         w->m++;
       }
     }
+    if ((rev + 1) % track.NrevMon == 0)
+      for (int k = 0; k < Nb; k++) {
+        const bunch_stats_t *s = &W[k].bunch.stats;
+        fprintf(famp, "%ld %d %.17g %.17g %.17g %.17g %.17g %.17g\n", rev, W[k].kb,
+                s->ampinv.v[0], s->ampinv.v[1], s->ampinv.v[2],
+                s->ampinv_cm.v[0], s->ampinv_cm.v[1], s->ampinv_cm.v[2]);
+      }
+#pragma omp parallel for schedule(dynamic, 1) if (par)
     for (int k = 0; k < Nb; k++) {
       worker_t *w = &W[k];
       if (nlrL > 0) fnp_ring_update(&w->ring, fnp_ring, &w->sf, &w->bunch, w->kb, LON);
@@ -374,6 +394,12 @@ int main(int argc, char **argv)
   }
   clock_gettime(CLOCK_MONOTONIC, &t_b);
   fprintf(stderr, "turn_loop_seconds %.6f\n", (t_b.tv_sec - t_a.tv_sec) + 1e-9 * (t_b.tv_nsec - t_a.tv_nsec));
+  if (serial_turns > 0 && serial_turns < track.NrevTot) {
+    fprintf(stderr, "serial_turns %ld seconds %.6f\n", serial_turns,
+            (t_s.tv_sec - t_a.tv_sec) + 1e-9 * (t_s.tv_nsec - t_a.tv_nsec));
+    fprintf(stderr, "parallel_turns %ld seconds %.6f threads %d\n", track.NrevTot - serial_turns,
+            (t_b.tv_sec - t_s.tv_sec) + 1e-9 * (t_b.tv_nsec - t_s.tv_nsec), qe_off ? omp_get_max_threads() : 1);
+  }
   if (track.scan == 0)
     for (int k = 0; k < Nb; k++) {
       weak_bunch_calc_statistics(&W[k].bunch, &track);

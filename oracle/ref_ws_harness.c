@@ -40,6 +40,7 @@
  * does (src/manager_weakweak.c:58-80); 0 keeps the reference's zeros.
  */
 #include <stdio.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -164,6 +165,8 @@ int main(int argc, char **argv)
   FILE *fz = fopen("zcm.txt", "w");
   (void)path;
   if (wfiles) printf("Starting multi bunch tracking...\n");
+  struct timespec t_a, t_b;
+  clock_gettime(CLOCK_MONOTONIC, &t_a);
   for (long rev = 0; rev < track.NrevTot; rev++) {
     if (wfiles && rev % 100 == 0) {
       if (rev + 99 >= track.NrevTot) printf("Transformation for turn %3ld to %3ld", rev, track.NrevTot - 1);
@@ -273,6 +276,8 @@ int main(int argc, char **argv)
     }
     fprintf(fz, "\n");
   }
+  clock_gettime(CLOCK_MONOTONIC, &t_b);
+  fprintf(stderr, "turn_loop_seconds %.6f\n", (t_b.tv_sec - t_a.tv_sec) + 1e-9 * (t_b.tv_nsec - t_a.tv_nsec));
   if (wfiles) {
     if (track.TrackPlane[HOR]) bunch_writeout_mean_ampinv_pos(track.NrevTot, track.NrevMon, mh, ebeam, HOR);
     if (track.TrackPlane[VER]) bunch_writeout_mean_ampinv_pos(track.NrevTot, track.NrevMon, mh, ebeam, VER);

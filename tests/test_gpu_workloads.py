@@ -16,8 +16,9 @@ work decompositions the benchmark runs them with.
   smallest and the largest.
 
 Bounds as in test_gpu_parity.py (centroids normalised by the rms, rms relative,
-RTOL = 1e-12; slice counts and trash counters bit-exact) and test_gpu_ws.py for
-the beam-ion train.
+RTOL = 1e-12; slice counts and trash counters bit-exact); the beam-ion train
+as test_gpu_ws.py measures it, against RTOL = 1e-12 (measured 5e-13, the cloud's
+charge sum over 365,000 ions in a tree order against the reference's loop).
 """
 import os
 

@@ -4,12 +4,19 @@ functions) on the same inputs.
 
 Sources of difference, all at the level of rounding: the device's sin in the
 synchrotron step and exp/sin/cos inside the Faddeeva function (within a few
-ulp of glibc's), and the summation order of the ion-field sums over the cloud
-(a block tree instead of a sequential loop).  Both forms of the ion train are
-checked: one lane per ion over every later bunch (the default) and the
-reference's bunch-by-bunch relay (MBT_WS_SERIAL_TRAIN).  Every output is compared as
-max |gpu - oracle| / max |oracle| over the run, per quantity, against RTOL.
-Runs are bitwise reproducible (fixed-order reductions).
+ulp of glibc's), the reciprocal grid spacing of the field interpolation, and
+the summation order of the ion-field sums over the cloud (a block tree instead
+of a sequential loop).  Both forms of the ion train are checked: one lane per
+ion over every later bunch (the default) and the reference's bunch-by-bunch
+relay (MBT_WS_SERIAL_TRAIN).  Every output is compared as max |gpu - oracle| /
+max |oracle| over the run, per quantity, against RTOL = 1e-12, except on the
+416-bucket configuration: there the ion cloud's Nri y sum (terms of both
+signs) and the quantities it feeds move by 1.1e-11 to 2.1e-11 under each
+rounding-level change alone in the oracle itself
+(tests/test_oracle_ws_sensitivity.py: reciprocal grid quotients, tree sums,
+one-ulp libm results), so the bound there is RTOL_H416 = 2e-11, the largest of
+those (the GPU measures 6.2e-12).  Runs are bitwise reproducible (fixed-order
+reductions).
 """
 import numpy as np
 import pytest
@@ -19,7 +26,8 @@ import oracle as o
 from variants import WS_VARIANTS, write_ws_variant
 
 pytestmark = pytest.mark.gpu
-RTOL = 1e-10
+RTOL = 1e-12
+RTOL_H416 = 2e-11
 
 
 def scaled_err(g, r):
@@ -50,7 +58,7 @@ def test_ws_matches_oracle(tmp_path, name, serial):
     for k, lab in enumerate(["tau", "delta", "x", "xp", "z", "zp"]):
         errs["slice_" + lab] = scaled_err(sl_g[:, k], sl_o[:, k]) if np.any(sl_o[:, k]) else float(np.max(np.abs(sl_g[:, k])))
     print(name, {k: "%.2g" % v for k, v in errs.items()}, flush=True)
-    assert max(errs.values()) <= RTOL, errs
+    assert max(errs.values()) <= (RTOL_H416 if name == "ws_h416" else RTOL), errs
     g.close()
     orc.close()
 
