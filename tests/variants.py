@@ -269,3 +269,87 @@ def write_inp(d, name="legacy"):
         for comment, data in INP_LINES:
             f.write(comment + "\n" + data + "\n")
     return path
+
+
+# ---- growth rates --------------------------------------------------------------
+# A uniformly filled ring with the resistive-wall long-range wake alone (no
+# short-range wake, chromaticity 0, quantum excitation and with it radiation
+# damping off), every particle of a bunch on the same point: rigid bunches whose
+# centroids follow the coupled-bunch modes exactly.
+RW_GROWTH = dict(h=40, nmlt=50, iring=500.0, beff=0.003, rho=1.7e-8, turns=4000, mode=1, amp=1e-6)
+
+
+def write_rw_growth(d, g=RW_GROWTH):
+    os.makedirs(d, exist_ok=True)
+    text = cg.ring_conf(sections={
+        "tracking": {"NrevTot": g["turns"], "NrevMon": 10, "EnableRW_long": 1, "Nmlt": g["nmlt"],
+                     "EnableQuantum": 0},
+        "ring": {"Lc": cg.small_ring(g["h"]), "Iring": g["iring"], "Cx": 0.0, "Cz": 0.0, "rhorw": g["rho"],
+                 "beffL1": g["beff"], "beffL2": g["beff"], "beffH3": g["beff"], "beffV3": g["beff"]},
+        "distribution": {"Np": 2, "xxCM_offset": 0.0, "zzCM_offset": 0.0}})
+    return cg.write(os.path.join(d, "rw_growth.conf"), text)
+
+
+def rw_growth_bunches(g=RW_GROWTH, beta=10.0):
+    """Initial bunches [h][6][2] of the growth ring: every particle at the origin
+    but for a transverse offset in the pattern of coupled-bunch mode mu in both
+    planes, x_k - i beta x'_k = a exp(2 pi i mu k / h)."""
+    import numpy as np
+    out = np.zeros((g["h"], 6, 2))
+    for k in range(g["h"]):
+        ph = 2 * np.pi * g["mode"] * k / g["h"]
+        for c in (2, 4):
+            out[k, c] = g["amp"] * np.cos(ph)
+            out[k, c + 1] = -g["amp"] / beta * np.sin(ph)
+    return out
+
+
+def rw_growth_analytic(E0, Lc, Iring, Q, beff, rho, beta=10.0):
+    """Growth rate [1/s] of the bunch action (twice the amplitude's) of the most
+    unstable transverse resistive-wall coupled-bunch mode: the closed form of the
+    reference's test, src/test_rw.c:10-27 (tau_to_minus1) doubled as at
+    :78-79, with the total current Iring [mA], E0 [GeV], Lc [m], the pipe radius
+    beff [m], the resistivity rho [Ohm m] and the fractional tune of Q."""
+    import numpy as np
+    w0 = 2.0 * np.pi * 2.997925e8 / Lc
+    eps0 = 8.854187817e-12
+    dq = np.fmod(Q, 1.0)
+    return 2.0 * (beta * w0 * Iring * 1e-3 / (4.0 * np.pi * E0 * 1e9)) * (Lc / (2.0 * np.pi) / beff ** 3) * \
+        np.sqrt(2.0 * rho / ((1.0 - dq) * w0 * eps0))
+
+
+def mode_growth(stats, mode, first, beta=10.0):
+    """Slope per turn of log |a_mu|^2 over the turns from `first` on, per
+    transverse plane, where a_mu is the discrete Fourier component mu over the
+    bunches of x - i beta x' (centroids from stats [turns][bunches][12])."""
+    import numpy as np
+    out = []
+    for p in (1, 2):
+        A = stats[:, :, p] - 1j * beta * stats[:, :, 6 + p]
+        F = np.fft.fft(A, axis=1)[:, mode]
+        t = np.arange(len(stats))
+        out.append(float(np.polyfit(t[first:], np.log(np.abs(F[first:]) ** 2), 1)[0]))
+    return out
+
+
+def write_c1_seed(d, seed, turns=1000, iring=400.0, beff=0.006):
+    """The small ring driven unstable (confgen.wl_c1 at iring mA, pipe radius
+    beff), quantum excitation on, with its three generator seeds moved by
+    10 seed."""
+    path = cg.wl_c1(d, turns=turns, iring=iring, beff=beff)
+    text = open(path).read()
+    for key, base in (("iseed_LON", 1001), ("iseed_HOR", 2002), ("iseed_VER", 3003)):
+        text = text.replace("%s = %d" % (key, base), "%s = %d" % (key, base + 10 * seed))
+    return cg.write(path, text)
+
+
+def action_growth(ampinv, first=500, mon=10):
+    """Slope per turn of log of the bunch-averaged centroid action (ampinv_cm)
+    over the monitor turns from `first` on, HOR and VER."""
+    import numpy as np
+    turns = np.arange(len(ampinv))
+    sel = (turns + 1) % mon == 0
+    a = ampinv[sel][..., 3:6].mean(1)
+    t = turns[sel]
+    s = t >= first
+    return [float(np.polyfit(t[s], np.log(a[s, p]), 1)[0]) for p in (1, 2)]
