@@ -111,6 +111,11 @@
 // reference's formulas so that every constant equals the reference's.
 struct DevParams {
   int ring, Nb, h, Nc, Nmlt;
+  // bound on |x| and |z| of a particle whose dipole enters the exact fixed-point
+  // slice sums: min(2^20, 2^27 / Np), so that the sum of a whole bunch stays
+  // inside the 128-bit range (|sum| < 2^27 at the quantum 2^-100); a particle
+  // beyond it marks the bunch's sums NaN
+  double fx_lim;
   unsigned long long zero_bits; // always 0: the timing pass's stores xor with it
   int mixed;                    // compensated FP32 wake convolutions (MBT_RING_MIXED)
   int bunch_off;  // first global bunch of the ring
@@ -267,34 +272,7 @@ __device__ __noinline__ int bin_of(double tau, double sgm, double Nsigma, double
   return (int)v;
 }
 
-// Exact 128-bit fixed point, quantum 2^-100: q = round(x 2^100) for |x| < 2^20.
-__device__ __forceinline__ void fx_split(double x, unsigned long long &lo, unsigned long long &hi)
-{
-  unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-  int e = (int)((bits >> 52) & 0x7ff);
-  unsigned long long m = (bits & 0xfffffffffffffULL) | (1ULL << 52);
-  lo = 0;
-  hi = 0;
-  if (e == 0) return;
-  int s = e - 975;
-  if (s >= 64) {
-    hi = m << (s - 64);
-  } else if (s > 0) {
-    lo = m << s;
-    hi = m >> (64 - s);
-  } else if (s == 0) {
-    lo = m;
-  } else if (s > -64) {
-    int r = -s;
-    lo = (m >> r) + ((m >> (r - 1)) & 1ULL);
-  } else if (s == -64) {
-    lo = (m >> 63) & 1ULL;
-  }
-  if (bits >> 63) { // negate
-    lo = ~lo + 1ULL;
-    hi = ~hi + (lo == 0ULL ? 1ULL : 0ULL);
-  }
-}
+#include "fixedpoint.h"
 
 __device__ __forceinline__ void fx_atomic_add(unsigned long long *plo, unsigned long long *phi, unsigned long long lo,
                                               unsigned long long hi)
@@ -302,13 +280,6 @@ __device__ __forceinline__ void fx_atomic_add(unsigned long long *plo, unsigned 
   unsigned long long old = atomicAdd(plo, lo);
   unsigned long long carry = (old + lo) < old ? 1ULL : 0ULL;
   atomicAdd(phi, hi + carry);
-}
-
-__device__ __forceinline__ double fx_to_double(unsigned long long lo, unsigned long long hi)
-{
-  long long h = (long long)hi;
-  double v = (double)h * 18446744073709551616.0 + (double)lo;
-  return v * 7.888609052210118e-31; // 2^-100
 }
 
 // ------------------------------------------------------------------ k_turn
@@ -695,7 +666,7 @@ __global__ __launch_bounds__(WG, TURN_WAVES_PER_EU) void k_turn(const DevParams 
       else atomicAdd(&ltc[bn < 0 ? 0 : 1], 1u);
       if (DIP) {
         unsigned long long xl, xh, zl, zh;
-        if (!(fabs(x) < 1048576.0) || !(fabs(z) < 1048576.0)) {
+        if (!(fabs(x) < P.fx_lim) || !(fabs(z) < P.fx_lim)) {
           lbad = 1;
           xl = xh = zl = zh = 0;
         } else {
@@ -1792,6 +1763,7 @@ static size_t turn_lds(int Nc)
 static void fill_params(const mbt_config *c, const int planes[3], DevParams &P)
 {
   memset(&P, 0, sizeof P);
+  P.fx_lim = c->Np > 128 ? 134217728.0 / (double)c->Np : 1048576.0;
   P.Nb = c->Nbunch;
   P.h = (int)c->Nharm;
   P.Nc = c->Ncell;
