@@ -48,8 +48,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_MP_TURN = 96.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations of one ion-slice interaction in the reference's train
+# (src/fbi_weakstrong.c:370-441 with get_linearly_interpolated_Efields :454-580):
+# two grid quotients and two fractions (8), six bilinear steps of a subtraction,
+# a multiplication and an addition (18), the kicks of the ion (10) and the
+# two field sums (4)
+FLOPS_PER_INTERACTION = 40
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_turn_production_traffic.txt")
 TRAFFIC_FILE_BATCHED = os.path.join(ROOT, "profiles", "k_turn_batched_traffic.txt")
+TRAFFIC_NOTE = ("%s: HBM bytes per k_turn launch from separate rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                "the same workload, scaled by the counters' reading on a plain device copy (not measured in this run)")
 
 
 def production_conf(d, np_, turns):
@@ -76,29 +85,54 @@ def read_traffic(path=TRAFFIC_FILE):
     return vals.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(np_, turns=2, timeout=240):
-    """Reference turn loop on the host (one core), same ring, `turns` turns."""
+def host_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS where set, else the
+    machine's cores, at most 16 (the host share of one GPU)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(np_, turns=20, serial_turns=2, timeout=300):
+    """The reference's own turn loop (oracle/_ref/refharness: its translation
+    units compiled from the reference sources) on the production ring: the
+    first `serial_turns` turns on one core, then `turns` turns with the bunches
+    spread over the host's cores (one thread per bunch at a time, as the
+    reference spreads them over MPI ranks).  Set-up is not timed.  Without the
+    harness, the project's C restatement on one core."""
     harness = os.path.join(ROOT, "oracle", "_ref", "refharness")
+    threads = host_threads()
     with tempfile.TemporaryDirectory() as d:
-        conf = production_conf(d, np_, turns)
-        sample = "production ring, 416 bunches x %d mp x %d turns (turn loop only)" % (np_, turns)
+        conf = production_conf(d, np_, serial_turns + turns)
         if os.path.exists(harness):
-            r = subprocess.run([harness, conf, d, str(turns)], capture_output=True, text=True, timeout=timeout)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            r = subprocess.run([harness, conf, d, str(serial_turns + turns), "0", "-1", str(serial_turns)],
+                               capture_output=True, text=True, timeout=timeout, env=env)
+            ser = par = None
             for line in r.stderr.splitlines():
-                if line.startswith("turn_loop_seconds"):
-                    sec = float(line.split()[1])
-                    return {"value": 416 * np_ * turns / sec, "unit": "macroparticle-turns/s", "cores": 1,
-                            "kind": "reference", "sample": sample}
+                p = line.split()
+                if p and p[0] == "serial_turns":
+                    ser = float(p[3])
+                elif p and p[0] == "parallel_turns":
+                    par, used = float(p[3]), int(p[5])
+            if r.returncode == 0 and ser and par:
+                mp = 416 * np_
+                return {"value": mp * turns / par, "unit": "macroparticle-turns/s", "cores": used,
+                        "kind": "reference",
+                        "sample": "production ring, 416 bunches x %d mp x %d turns, bunches over %d threads "
+                                  "(turn loop only)" % (np_, turns, used),
+                        "one_core": {"value": mp * serial_turns / ser, "cores": 1,
+                                     "sample": "the same ring, %d turns on one thread" % serial_turns}}
         import mtrack_amd as m
         import oracle
         cfg = m.Config(conf, d + "/")
         orc = oracle.OracleRing(cfg)
         orc.prepare()
         t0 = time.perf_counter()
-        orc.track(turns)
+        orc.track(serial_turns)
         sec = time.perf_counter() - t0
-        return {"value": 416 * np_ * turns / sec, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
-                "sample": sample}
+        return {"value": 416 * np_ * serial_turns / sec, "unit": "macroparticle-turns/s", "cores": 1,
+                "kind": "port", "sample": "production ring, 416 bunches x %d mp x %d turns" % (np_, serial_turns)}
 
 
 def stat_errors(g, r):
@@ -127,12 +161,33 @@ def growth_rates(am, nrev_mon, first):
     return [float(np.polyfit(t[sel], np.log(a[sel, p]), 1)[0]) for p in (1, 2)]
 
 
+def split_errors(g, r):
+    """Errors of per-bunch statistics [..., 12] against a reference run: centroids
+    (position and slope) relative to the reference rms of the same coordinate,
+    rms relative to itself, and the emittance proxy rms(pos) * rms(slope)
+    relative to itself, each the largest over bunches, turns and planes."""
+    import numpy as np
+    g, r = np.asarray(g), np.asarray(r)
+    cen = rms = emit = 0.0
+    for p in range(3):
+        for mc, sc in ((p, 3 + p), (6 + p, 9 + p)):
+            scale = np.maximum(np.abs(r[..., sc]), 1e-300)
+            cen = max(cen, float(np.max(np.abs(g[..., mc] - r[..., mc]) / scale)))
+            rms = max(rms, float(np.max(np.abs(g[..., sc] - r[..., sc]) / scale)))
+        eg, er = g[..., 3 + p] * g[..., 9 + p], r[..., 3 + p] * r[..., 9 + p]
+        emit = max(emit, float(np.max(np.abs(eg - er) / np.maximum(np.abs(er), 1e-300))))
+    return {"centroid": cen, "rms": rms, "emittance": emit}
+
+
 def c1(turns=2000):
     """The small multi-bunch ring (64 bunches x 1,000 macroparticles, QE on,
     copper RW, broadband resonator; confgen.wl_c1): the plain FP64 GPU path and
-    the mixed-precision one, each timed over `turns` turns from the same start,
-    with the mixed run's errors against FP64; growth rates of the same ring
-    driven unstable (400 mA, 6 mm RW radius, 1,000 turns) in both precisions."""
+    the mixed-precision one (FP32 hi/lo-compensated wake convolutions and the
+    long-range resistive-wall sum as an FP32 hi/lo MFMA GEMM, MBT_RING_RW_GEMM),
+    each timed over `turns` turns from the same start, with the mixed run's
+    centroid, rms and emittance errors against FP64; growth rates of the same
+    ring driven unstable (400 mA, 6 mm RW radius, 1,000 turns) in both
+    precisions."""
     import mtrack_amd as m
     from mtrack_amd import confgen
     out = {}
@@ -141,19 +196,20 @@ def c1(turns=2000):
         n = 64 * cfg.geti("Np")
         st = {}
         for mixed in (False, True):
-            g = m.Ring(cfg, mixed=mixed)
+            g = m.Ring(cfg, mixed=mixed, force_gemm=mixed)
             g.prepare()
+            g.track(10)  # untimed: the turn graphs
             t0 = time.perf_counter()
             st[mixed], _, _ = g.track(turns)
             sec = time.perf_counter() - t0
             g.close()
             out["mixed" if mixed else "fp64"] = {"value": n * turns / sec, "unit": "macroparticle-turns/s",
                                                  "ms_per_step": 1e3 * sec / turns}
-        out["mixed"]["stat_error_vs_fp64"] = stat_errors(st[True], st[False])
+        out["mixed"]["error_vs_fp64"] = split_errors(st[True], st[False])
         cfg = m.Config(confgen.wl_c1(d, turns=1000, iring=400.0, beff=0.006), d + "/")
         rates = {}
         for mixed in (False, True):
-            g = m.Ring(cfg, mixed=mixed)
+            g = m.Ring(cfg, mixed=mixed, force_gemm=mixed)
             g.prepare()
             _, am, _ = g.track(1000)
             g.close()
@@ -164,10 +220,35 @@ def c1(turns=2000):
     out["steps"] = turns
     out["config"] = {"workload": "64 bunches (buckets int(6.5 k) of h 416) x 1,000 macroparticles, QE on, "
                                  "copper RW 12 mm over 19 turns, 10 GHz broadband resonator in 50 slices, 208 mA",
-                     "precision": "mixed: hi/lo-compensated FP32 wake convolutions (a lone ring keeps its "
-                                  "latency-bound per-bunch FP64 RW sum; ensembles use the FP32 hi/lo MFMA RW "
-                                  "GEMM); coordinates, phase advance and RF recurrence FP64"}
+                     "precision": "mixed: hi/lo-compensated FP32 wake convolutions and the long-range "
+                                  "resistive-wall sum as an FP32 hi/lo MFMA GEMM; coordinates, phase advance and "
+                                  "RF recurrence FP64"}
     return out
+
+
+def one_ring_at_a_time(nrings, seed, steps, warmup):
+    """The batched workload's rings tracked one after another, each alone through
+    the plain FP64 single-ring path (its own default decomposition and graphs):
+    the sum over rings of each ring's timed turns."""
+    import mtrack_amd as m
+    from mtrack_amd import confgen
+    total_s, total_mp = 0.0, 0
+    with tempfile.TemporaryDirectory() as d:
+        for p in confgen.wl_ensemble(d, nrings=nrings, seed=seed):
+            cfg = m.Config(p, d + "/")
+            g = m.Ring(cfg)
+            g.prepare()
+            if warmup > 0:
+                g.track(warmup)
+            g.track(steps, want_amp=False)  # untimed: the graph of a `steps`-turn block
+            t0 = time.perf_counter()
+            g.track(steps, want_amp=False)
+            total_s += time.perf_counter() - t0
+            total_mp += g.Nb * g.Np
+            g.close()
+    return {"value": total_mp * steps / total_s, "unit": "macroparticle-turns/s", "dtype": "f64", "rings": nrings,
+            "steps": steps, "ms_per_step": 1e3 * total_s / steps,
+            "note": "each ring alone through the single-ring FP64 path, timed turns summed over the rings"}
 
 
 def placement(achieved, ceiling):
@@ -190,13 +271,13 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
     with tempfile.TemporaryDirectory() as d:
         paths = confgen.wl_ensemble(d, nrings=nrings, seed=seed)
         cfgs = [m.Config(p, d + "/") for p in paths]
-        ens = m.Ensemble(cfgs, init_dist=False, mixed=mixed)
-        # every bunch of a ring starts from the ring's first bunch as the
-        # reference generator makes it (the same Gaussian for all its bunches)
-        for k, c in enumerate(cfgs):
-            b0 = c.generate_bunch(c.buckets[0])
-            for g in range(ens.first[k], ens.first[k + 1]):
-                ens.set_bunch(g, b0)
+        # every bunch from the reference's generator, seeded per ring and bucket
+        # (made on the device, bit for bit the host generator's)
+        c0 = time.perf_counter()
+        ens = m.Ensemble(cfgs, mixed=mixed)
+        create_s = time.perf_counter() - c0
+        tried, chosen = ens.placement()
+        search = {"candidates": tried, "chosen_GB/s": chosen}
         ens.prepare()
         R, G, N = ens.size()
         if chunk:
@@ -208,9 +289,11 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
             ens.track(warmup)
         ens.track(steps)  # untimed: instantiates the graph of a `steps`-turn block
         # timed region: the default path, the turns replayed as HIP graphs
+        builds0 = ens.graph_builds()
         t0 = time.perf_counter()
         last_stats, _, _ = ens.track(steps)
         t1 = time.perf_counter()
+        builds_timed = ens.graph_builds() - builds0
         # instrumented pass: the same number of turns launched eagerly with HIP
         # events around every kernel class (per-kernel times, roofline)
         ens.set_timing(True)
@@ -232,6 +315,8 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
             rows, tmon = [], []
             start = warmup + 3 * steps  # turns tracked so far
             for blk in range(growth_turns // mon):
+                if blk % 10 == 0:
+                    print("growth-rate turns: %d of %d" % (blk * mon, growth_turns), file=sys.stderr, flush=True)
                 _, am, _ = ens.track(mon, want_stats=False, want_rf=False)
                 r = (mon - 1 - start % mon) % mon  # the block's monitor turn
                 rows.append(am[r].copy())
@@ -250,7 +335,7 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
     achieved = bytes_turn / (k_turn_ms * 1e-3) / 1e9
     return {"value": N * steps / sec, "unit": "macroparticle-turns/s", "dtype": "mixed" if mixed else "f64",
             "steps": steps,
-            "warmup": warmup, "ms_per_step": 1e3 * sec / steps,
+            "warmup": warmup, "ms_per_step": 1e3 * sec / steps, "graphs_built_in_timed_region": builds_timed,
             "config": {"workload": "batched ensemble of %d independent rings in one launch per kernel: bunches "
                                    "32-416 and macroparticles 1e4-1e5 log-uniform, 0.1-5 mA/bunch, chromaticity "
                                    "-1..3, 100 slices, broadband resonator + resistive wall 50 turns + main cavity "
@@ -260,10 +345,11 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
             "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                    "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(TRAFFIC_FILE_BATCHED)},
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(TRAFFIC_FILE_BATCHED),
+                         "traffic_source": TRAFFIC_NOTE % os.path.relpath(TRAFFIC_FILE_BATCHED, ROOT)},
             "instrumented": {"note": "same number of turns launched eagerly, HIP events around every kernel class",
                              "ms_per_step": 1e3 * (e1 - e0) / steps, "value": N * steps / (e1 - e0)},
-            "placement_ceiling": placement(achieved, ceiling),
+            "placement_ceiling": dict(placement(achieved, ceiling), search=search), "create_s": create_s,
             **({"growth_rates": growth} if growth else {})}, last_stats
 
 
@@ -302,26 +388,42 @@ def fbii(steps, warmup, cpu=True):
         t0 = time.perf_counter()
         w.track(steps)
         t1 = time.perf_counter()
+        # instrumented pass: HIP events around the ion train of every turn
+        w.set_timing(True)
+        w.track(steps)
+        train_ms, trains = w.train_time()
         w.close()
     sec = (t1 - t0) / steps
+    ions_ms = train_ms / max(trains, 1)
+    tflops = ion_slice_updates(nb) * FLOPS_PER_INTERACTION / (ions_ms * 1e-3) / 1e12
     out = {"value": ion_slice_updates(nb) / sec, "unit": "ion-slice interactions/s", "dtype": "f64",
            "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * sec,
            "config": {"workload": "fast beam-ion weak-strong: h 416, %d-bunch train + %d-bucket gap, 41 slice "
                                   "centroids and 1,025 macro-ions per bunch passage, Bassetti-Erskine fields, "
                                   "bunch currents of the filling" % (nb, 416 - nb),
-                      "bunches": nb, "ion_slice_interactions_per_turn": ion_slice_updates(nb)}}
+                      "bunches": nb, "ion_slice_interactions_per_turn": ion_slice_updates(nb)},
+           "kernel_ms_per_turn": {"k_ws_ions": ions_ms},
+           "roofline": {"bound": "fp64-valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                        "note": "k_ws_ions: %d FP64 operations per ion-slice interaction (the reference's loop, "
+                                "src/fbi_weakstrong.c:370-441,454-580), none of them fused; duration from HIP "
+                                "events around the kernel on its stream" % FLOPS_PER_INTERACTION}}
     if cpu:
+        # the reference's beam-ion turn (oracle/_ref/refwsharness), turn loop only,
+        # on a 128-bunch train: its train is a sequential relay through the
+        # manager (one core), and its cost per ion-slice interaction does not
+        # depend on the train length (interactions grow as the square of it)
         harness = os.path.join(ROOT, "oracle", "_ref", "refwsharness")
         if os.path.exists(harness):
             with tempfile.TemporaryDirectory() as d:
-                conf = fbii_conf(d, train=48, turns=1)
-                t0 = time.perf_counter()
+                conf = fbii_conf(d, train=128, turns=1)
                 r = subprocess.run([harness, conf, d, "1", "1"], capture_output=True, text=True, timeout=300)
-                t1 = time.perf_counter()
-                if r.returncode == 0:
-                    out["cpu_baseline"] = {"value": ion_slice_updates(48) / (t1 - t0), "unit": "ion-slice interactions/s",
-                                           "cores": 1, "kind": "reference",
-                                           "sample": "48-bunch train, 1 turn, set-up included"}
+                sec = [float(l.split()[1]) for l in r.stderr.splitlines() if l.startswith("turn_loop_seconds")]
+                if r.returncode == 0 and sec:
+                    out["cpu_baseline"] = {"value": ion_slice_updates(128) / sec[0],
+                                           "unit": "ion-slice interactions/s", "cores": 1, "kind": "reference",
+                                           "sample": "128-bunch train + 288-bucket gap, 1 turn, turn loop only "
+                                                     "(the reference relays the ion cloud through one process)"}
     return out
 
 
@@ -344,7 +446,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--np", type=int, default=100000, dest="np_")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-turns", type=int, default=2)
+    ap.add_argument("--cpu-turns", type=int, default=20, help="turns of the multi-core CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly (no HIP graphs)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble line")
@@ -355,8 +457,10 @@ def main():
                     help="work items per lane for the batched line (0: autotune)")
     ap.add_argument("--batched-mixed", action="store_true", help="with --no-production: the batched line in mixed precision")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision batched run")
-    ap.add_argument("--batched-growth-turns", type=int, default=0,
+    ap.add_argument("--batched-growth-turns", type=int, default=2000,
                     help="after the timed turns, track this many more and report per-ring growth rates")
+    ap.add_argument("--no-one-at-a-time", action="store_true",
+                    help="skip timing the batched rings one at a time through the single-ring path")
     ap.add_argument("--no-c1", action="store_true", help="skip the small-ring FP64 / mixed line")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production line (profiling of the other lines only)")
@@ -392,9 +496,11 @@ def main():
         ring.track(a.steps)  # untimed: instantiates the graph of a `steps`-turn block
         # timed region: the default path, the turns replayed as HIP graphs (each
         # mbt_ring_track call ends with a stream synchronisation)
+        builds0 = ring.graph_builds()
         t0 = time.perf_counter()
         ring.track(a.steps)
         t1 = time.perf_counter()
+        builds_timed = ring.graph_builds() - builds0
         # instrumented pass: the same number of turns launched eagerly with every
         # kernel class bracketed by HIP events on the ring's stream
         ring.set_timing(True)
@@ -430,9 +536,11 @@ def main():
         "kernel_ms_per_turn": {"k_turn": k_turn_ms, "k_reduce": ms[1] / max(launches[1], 1),
                                "k_lr": ms[2] / max(launches[2], 1), "k_rw": ms[3] / max(launches[3], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "traffic_source": TRAFFIC_NOTE % os.path.relpath(TRAFFIC_FILE, ROOT)},
         "timed_path": "HIP graph replay of the turns (the library's default)" if not a.no_graphs
                       else "eager launches (--no-graphs)",
+        "graphs_built_in_timed_region": builds_timed,
         "instrumented": {"note": "same number of turns launched eagerly, HIP events around every kernel class",
                          "ms_per_step": 1e3 * (e1 - e0) / a.steps, "value": nb * npb * a.steps / (e1 - e0)},
         "placement_ceiling": placement(achieved, ceiling),
@@ -449,12 +557,16 @@ def main():
             if not a.no_mixed:
                 mix, st32 = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed,
                                     out["batched"]["config"]["chunk_items"], mixed=True)
-                mix["stat_error_vs_fp64"] = stat_errors(st32, st64)
-                if isinstance(out.get("c1"), dict) and out["c1"].get("fp64"):
-                    mix["vs_plain_fp64_c1"] = mix["value"] / out["c1"]["fp64"]["value"]
+                mix["error_vs_fp64"] = split_errors(st32, st64)
                 for k in ("config", "roofline"):
                     mix.pop(k, None)
                 out["batched"]["mixed"] = mix
+            if not a.no_one_at_a_time:
+                single = one_ring_at_a_time(a.batched_rings, a.batched_seed, a.batched_steps, 3)
+                out["batched"]["one_ring_at_a_time_fp64"] = single
+                out["batched"]["vs_one_ring_at_a_time"] = out["batched"]["value"] / single["value"]
+                if "mixed" in out["batched"]:
+                    out["batched"]["mixed"]["vs_one_ring_at_a_time"] = out["batched"]["mixed"]["value"] / single["value"]
         except Exception as e:  # reported beside the FP64 line, never replaces it
             out["batched"] = {"value": None, "error": str(e)[:300]}
     if not a.no_fbii:

@@ -257,6 +257,11 @@ int mbt_ring_rw_history(const mbt_ring *r, double *out);
 int mbt_ring_kernel_times(const mbt_ring *r, double *ms, long *launches);
 /* Bytes of HBM the turn map pass must move per turn (algorithmic). */
 double mbt_ring_bytes_per_turn(const mbt_ring *r);
+/* No counterpart in the reference (a measurement aid): how many allocations of
+ * the coordinates the set-up tried (rings of 16 GB of coordinates and more try
+ * up to 3 and keep the fastest for the turn pass's traffic) and the in-place
+ * stream rate in GB/s of the one kept (0 when the ring is too small to search). */
+int mbt_ring_placement(const mbt_ring *r, int *candidates, double *gbps);
 /* No counterpart in the reference (a measurement aid): how many turn graphs
  * (one per block of turns, reused for later blocks whose first monitor turn
  * sits at the same offset) the ring has captured and instantiated so far. */
@@ -344,6 +349,12 @@ long mbt_ws_turn(const mbt_ws *w);
  * every bucket slices[h][6][S], npop[S], grid counts nofe[S][5][5], dist[6][Np] */
 void mbt_ws_grid(const mbt_config *cfg, double *agLOCx, double *agLOCz);
 int mbt_ws_generate(const mbt_config *cfg, double *slices, int *npop, int *nofe, double *dist);
+/* No counterpart in the reference (measurement aids): HIP events on the object's
+ * stream around the ion train of every turn (k_ws_ions, or the relay's launches)
+ * while on; mbt_ws_train_time returns the summed milliseconds and the number of
+ * trains of the last mbt_ws_track call. */
+int mbt_ws_set_timing(mbt_ws *w, int on);
+int mbt_ws_train_time(const mbt_ws *w, double *ms, long *launches);
 /* manager_weakstrong + its workers on one GPU: the reference's files in cfg->work_path */
 int mbt_manager_weakstrong(const mbt_config *cfg, int device, int flags);
 

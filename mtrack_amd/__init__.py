@@ -77,6 +77,7 @@ def lib():
         L.mbt_ring_stream_ceiling.argtypes = [vp, ctypes.c_int, dp]
         L.mbt_ring_rf_state.argtypes = [vp, ip, dp]
         L.mbt_ring_graph_builds.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
+        L.mbt_ring_placement.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
         L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
@@ -85,6 +86,8 @@ def lib():
         L.mbt_ws_distribution.argtypes = [vp, dp]
         L.mbt_ws_generate.argtypes = [vp, dp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp]
         L.mbt_manager_weakstrong.argtypes = [vp, ip, ip]
+        L.mbt_ws_set_timing.argtypes = [vp, ip]
+        L.mbt_ws_train_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
         if hasattr(L, "mbt_manager_weakweak"):
             L.mbt_manager_weakweak.argtypes = [vp, ip]
         if hasattr(L, "mbt_wofz"):
@@ -249,6 +252,12 @@ class Ring:
         _check(lib().mbt_ring_graph_builds(self.ptr, ctypes.byref(n)))
         return n.value
 
+    def placement(self):
+        """(coordinate allocations tried at set-up, stream GB/s of the one kept)"""
+        n, g = ctypes.c_int(), ctypes.c_double()
+        _check(lib().mbt_ring_placement(self.ptr, ctypes.byref(n), ctypes.byref(g)))
+        return n.value, g.value
+
     def stream_ceiling(self, reps=10):
         """GB/s of a plain in-place update of the ring's own coordinate arrays
         (bit-preserving): the ceiling their placement sets for the turn pass."""
@@ -352,6 +361,15 @@ class WeakStrong:
         zcm = np.zeros((nturns, self.h))
         _check(lib().mbt_ws_track(self.ptr, nturns, dptr(cm), dptr(mon), dptr(ion), dptr(zcm)))
         return cm, mon, ion, zcm
+
+    def set_timing(self, on=True):
+        _check(lib().mbt_ws_set_timing(self.ptr, 1 if on else 0))
+
+    def train_time(self):
+        """(milliseconds summed over the ion trains of the last track call, trains)"""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib().mbt_ws_train_time(self.ptr, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def slices(self, ib):
         out = np.zeros(6 * self.S)

@@ -1740,6 +1740,8 @@ struct mbt_ring {
   GraphEntry graphs[NGRAPH];
   int graph_next, use_graphs;
   long graph_builds; // turn graphs captured and instantiated so far
+  int place_candidates; // coordinate allocations tried at set-up
+  double place_gbps;    // k_stream rate of the chosen one (0 when not searched)
   LRConst *lc;
   int *lr_first; // [R+1] first resonator of each ring in lc
   int nres;
@@ -2139,6 +2141,126 @@ static int setup_rw_split(mbt_ring *r)
   return MBT_OK;
 }
 
+// Plain in-place update of the six coordinate arrays, every bit written back
+// (xor with a zero the compiler cannot see): the rate the arrays' placement
+// allows for k_turn's loads and stores, without its work.  2,048 slots per block.
+__global__ __launch_bounds__(WG) void k_stream(Coords C, long long n, unsigned long long zero)
+{
+  const long long i0 = (long long)blockIdx.x * 2048;
+  const long long i1 = i0 + 2048 < n ? i0 + 2048 : n;
+  double *a[6] = {C.tau, C.del, C.x, C.xp, C.z, C.zp};
+  for (long long i = i0 + threadIdx.x; i < i1; i += WG) {
+    double v[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = LDC(a[c] + i);
+#pragma unroll
+    for (int c = 0; c < 6; c++) STC(__longlong_as_double(__double_as_longlong(v[c]) ^ (long long)zero), a[c] + i);
+  }
+}
+
+// Placement of the coordinates.  For working sets of tens of GB the six-array
+// in-place stream of the turn pass runs at either 5.3-5.5 or 6.0-6.4 TB/s,
+// depending on the allocation and not on the arrays' relative offsets; which
+// one a fresh allocation gets depends on the allocator's state (on fresh
+// devices, reserving and releasing the whole footprint first gave the fast
+// placement 4 times in 4; later allocations in a process land either way).
+// Working sets of a few GB stream at 5.4-5.6 TB/s wherever they lie (every
+// 2 GB window of three 24 GB allocations measured 5.44-5.62), so only large
+// footprints search: candidates are allocated while the device has room for
+// one more (at most PLACE_CANDIDATES), each timed with k_stream, the first at
+// PLACE_GOOD_GBS or above kept at once, else the fastest; the others are freed.
+#define PLACE_SEARCH_MIN (16ULL << 30)
+#define PLACE_CANDIDATES 3
+#define PLACE_GOOD_GBS 6000.0
+
+static int alloc_six(Coords &c, long long n)
+{
+  double **a[6] = {&c.tau, &c.del, &c.x, &c.xp, &c.z, &c.zp};
+  for (int k = 0; k < 6; k++) *a[k] = nullptr;
+  for (int k = 0; k < 6; k++)
+    if (hipMalloc((void **)a[k], (size_t)n * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      for (int j = 0; j < k; j++) (void)hipFree(*a[j]);
+      for (int j = 0; j < 6; j++) *a[j] = nullptr;
+      return MBT_ENOMEM;
+    }
+  return MBT_OK;
+}
+
+static void free_six(Coords &c)
+{
+  double *a[6] = {c.tau, c.del, c.x, c.xp, c.z, c.zp};
+  for (int k = 0; k < 6; k++)
+    if (a[k]) (void)hipFree(a[k]);
+  c = Coords{};
+}
+
+// GB/s of k_stream over one candidate's arrays (one untimed pass, one timed)
+static double time_six(hipStream_t st, const Coords &c, long long n)
+{
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return 0.0;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return 0.0;
+  }
+  const unsigned nb = (unsigned)((n + 2047) / 2048);
+  hipLaunchKernelGGL(k_stream, dim3(nb), dim3(WG), 0, st, c, n, 0ULL);
+  (void)hipEventRecord(e0, st);
+  hipLaunchKernelGGL(k_stream, dim3(nb), dim3(WG), 0, st, c, n, 0ULL);
+  (void)hipEventRecord(e1, st);
+  float ms = 0.0f;
+  double gbps = 0.0;
+  if (hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+      hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.0f)
+    gbps = 96.0 * (double)n / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return gbps;
+}
+
+static int place_coords(mbt_ring *r, long long n)
+{
+  const size_t foot = (size_t)n * 48;
+  if (foot >= COORD_RESERVE_MIN) {
+    void *reserve = NULL;
+    if (hipMalloc(&reserve, foot) == hipSuccess)
+      (void)hipFree(reserve);
+    else
+      (void)hipGetLastError(); // not enough memory to reserve: allocate the arrays as they come
+  }
+  Coords cand[PLACE_CANDIDATES];
+  double rate[PLACE_CANDIDATES];
+  int nc = 0, best = 0;
+  for (int k = 0; k < PLACE_CANDIDATES; k++) {
+    if (k > 0) {
+      size_t fr = 0, tot = 0;
+      if (foot < PLACE_SEARCH_MIN || hipMemGetInfo(&fr, &tot) != hipSuccess || fr < foot + (4ULL << 30)) break;
+    }
+    if (alloc_six(cand[k], n) != MBT_OK) {
+      if (k == 0) {
+        mbt_set_error("hipMalloc of the coordinates (%zu bytes) failed", foot);
+        return MBT_ENOMEM;
+      }
+      break;
+    }
+    nc = k + 1;
+    if (foot < PLACE_SEARCH_MIN) break;
+    rate[k] = time_six(r->stream, cand[k], n);
+    if (rate[k] > rate[best]) best = k;
+    if (rate[k] >= PLACE_GOOD_GBS) break;
+  }
+  for (int k = 0; k < nc; k++)
+    if (k != best) free_six(cand[k]);
+  r->C = cand[best];
+  r->place_candidates = nc;
+  r->place_gbps = nc > 1 || foot >= PLACE_SEARCH_MIN ? rate[best] : 0.0;
+  double *a[6] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp};
+  for (int k = 0; k < 6; k++) CK(hipMemsetAsync(a[k], 0, (size_t)n * sizeof(double), r->stream));
+  CK(hipStreamSynchronize(r->stream));
+  return MBT_OK;
+}
+
 // ---- initial distributions on the device -------------------------------------
 // The reference's Gaussian bunch generator (weak_generate_bunch_distribution,
 // src/bunch_weak.c:85-105, with c_fnorm, src/bunch.c:379-405) restated for the
@@ -2359,25 +2481,13 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
       }
     }
   }
-  // Placement: on a device whose memory has not been allocated before, six
-  // coordinate arrays allocated straight away streamed about 10 % slower
-  // (25.6-25.8 ms against 22.5-23.6 ms per turn of the 67.5 GB ensemble, on
-  // fresh devices, for the life of the allocation; physically contiguous
-  // allocations did not help).  Reserving the whole footprint once and
-  // releasing it before allocating the arrays gives them the fast placement.
-  if ((size_t)Nalloc * 48 >= COORD_RESERVE_MIN) {
-    void *reserve = NULL;
-    if (hipMalloc(&reserve, (size_t)Nalloc * 48) == hipSuccess)
-      (void)hipFree(reserve);
-    else
-      (void)hipGetLastError(); // not enough memory to reserve: allocate the arrays as they come
+  {
+    const int rcp = place_coords(r, Nalloc);
+    if (rcp) {
+      mbt_ring_destroy(r);
+      return rcp;
+    }
   }
-  DA(r->C.tau, Nalloc);
-  DA(r->C.del, Nalloc);
-  DA(r->C.x, Nalloc);
-  DA(r->C.xp, Nalloc);
-  DA(r->C.z, Nalloc);
-  DA(r->C.zp, Nalloc);
   DA(r->wg_first, nwg);
   DA(r->wg_end, nwg);
   DA(r->wg_bunch, nwg);
@@ -2981,6 +3091,17 @@ static int turn_graph(mbt_ring *r, long n, hipGraphExec_t *out)
   return MBT_OK;
 }
 
+extern "C" int mbt_ring_placement(const mbt_ring *r, int *candidates, double *gbps)
+{
+  if (!r || !candidates || !gbps) {
+    mbt_set_error("mbt_ring_placement: null argument");
+    return MBT_EINVAL;
+  }
+  *candidates = r->place_candidates;
+  *gbps = r->place_gbps;
+  return MBT_OK;
+}
+
 extern "C" int mbt_ring_graph_builds(const mbt_ring *r, long *count)
 {
   if (!r || !count) {
@@ -3150,23 +3271,6 @@ extern "C" int mbt_ring_set_timing(mbt_ring *r, int on)
   }
   r->timing = on;
   return MBT_OK;
-}
-
-// Plain in-place update of the six coordinate arrays, every bit written back
-// (xor with a zero the compiler cannot see): the rate the arrays' placement
-// allows for k_turn's loads and stores, without its work.  2,048 slots per block.
-__global__ __launch_bounds__(WG) void k_stream(Coords C, long long n, unsigned long long zero)
-{
-  const long long i0 = (long long)blockIdx.x * 2048;
-  const long long i1 = i0 + 2048 < n ? i0 + 2048 : n;
-  double *a[6] = {C.tau, C.del, C.x, C.xp, C.z, C.zp};
-  for (long long i = i0 + threadIdx.x; i < i1; i += WG) {
-    double v[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) v[c] = LDC(a[c] + i);
-#pragma unroll
-    for (int c = 0; c < 6; c++) STC(__longlong_as_double(__double_as_longlong(v[c]) ^ (long long)zero), a[c] + i);
-  }
 }
 
 // The placement ceiling of the ring's turn pass: k_stream over the ring's own

@@ -191,22 +191,19 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 __device__ __forceinline__ void ws_interp(const double2 *t, double aAx, double aAz, double iax, double iaz, double xx,
                                           double yy, double &Ex, double &Ey)
 {
-  // The reference's two sign cases are mirror images: a non-negative coordinate
-  // interpolates from cell k1 = trunc(x / d) towards +2, a negative one from k1
-  // towards -2, both clamped to the grid.  So the cell is found on |x| (trunc is
-  // sign-symmetric, as is every rounding below) and mirrored: with
-  // a = min(trunc(|x| / d), 2), k1 = s a and k2 = s min(a + 1, 2) for the sign
-  // s, and the signed fraction s (x - k1 d) / d equals (|x| - a d) / d exactly.
-  // Grid point (kx, kz), kx, kz in [-2, 2], is entry 12 + 5 kx + kz; the
-  // partner is one step further out in each plane (none at the clamped edge).
-  const int ax = min((int)(fabs(xx) * iax), 2), az = min((int)(fabs(yy) * iaz), 2);
-  const bool px = xx >= 0, pz = yy >= 0;
-  const int ox = 5 * ax, sx5 = ax < 2 ? 5 : 0, sz1 = az < 2 ? 1 : 0;
-  const int i1 = 12 + (px ? ox : -ox) + (pz ? az : -az);
-  const int i2 = i1 + (px ? sx5 : -sx5);
-  const int dz = pz ? sz1 : -sz1;
-  const double2 e1 = t[i1], e2 = t[i2], e3 = t[i1 + dz], e4 = t[i2 + dz];
-  const double sx = (fabs(xx) - ax * aAx) * iax, sz = (fabs(yy) - az * aAz) * iaz;
+  // Grid point (kx, kz), kx, kz in [-2, 2], is entry 5 kx + kz + 12.  The cell
+  // is k1 = trunc(x / d) clamped to the grid, its partner one step further from
+  // the centre in the direction of the coordinate's sign (the same point at the
+  // clamped edge), as in the reference's two sign cases.  The fraction is
+  // |x - k1 d| / d: the reference's fraction, negated where it interpolates
+  // towards the negative side (the sign flips are exact).
+  const int k1x = min(max((int)(xx * iax), -2), 2);
+  const int k1z = min(max((int)(yy * iaz), -2), 2);
+  const int k2x = min(max(k1x + (xx >= 0 ? 1 : -1), -2), 2);
+  const int k2z = min(max(k1z + (yy >= 0 ? 1 : -1), -2), 2);
+  const double2 *tc = t + 12;
+  const double2 e1 = tc[5 * k1x + k1z], e2 = tc[5 * k2x + k1z], e3 = tc[5 * k1x + k2z], e4 = tc[5 * k2x + k2z];
+  const double sx = fabs((xx - k1x * aAx) * iax), sz = fabs((yy - k1z * aAz) * iaz);
   const double y5 = e1.y + sx * (e2.y - e1.y);
   const double y6 = e3.y + sx * (e4.y - e3.y);
   const double x5 = e1.x + sx * (e2.x - e1.x);
@@ -731,6 +728,10 @@ struct mbt_ws {
   int serial;          // bunch-by-bunch relay instead of the ion-parallel train
   double *d_dtime;
   long long *d_toff;
+  int timing;         // HIP events around the ion train of every turn
+  hipEvent_t *ev;     // [2 WS_T] train start / end per turn of a block
+  double train_ms;    // summed train time of the last mbt_ws_track call
+  long train_launches;
 };
 
 extern "C" void mbt_ws_destroy(mbt_ws *w)
@@ -742,6 +743,9 @@ extern "C" void mbt_ws_destroy(mbt_ws *w)
                w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff};
   for (size_t i = 0; i < sizeof p / sizeof p[0]; i++)
     if (p[i]) (void)hipFree(p[i]);
+  if (w->ev)
+    for (int i = 0; i < 2 * WS_T; i++) (void)hipEventDestroy(w->ev[i]);
+  free(w->ev);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   free(w->bucket_h);
   free(w->fill_h);
@@ -999,6 +1003,8 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
   const int G = w->P.G, H = w->P.h;
   const size_t lds2 = sizeof(double) * 2 * H;
   long done = 0;
+  w->train_ms = 0.0;
+  w->train_launches = 0;
   while (done < nturns) {
     long n = nturns - done;
     if (n > WS_T) n = WS_T;
@@ -1011,9 +1017,11 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
       CK(hipGetLastError());
       hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
       CK(hipGetLastError());
+      if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
       if (w->P.fbii && !w->serial) {
         hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), 0, w->stream, w->P, w->D);
         CK(hipGetLastError());
+        if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
         hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
         CK(hipGetLastError());
       } else if (w->P.fbii) {
@@ -1027,10 +1035,20 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
           hipLaunchKernelGGL(k_ws_train, dim3(nblk), dim3(WS_WG), 0, w->stream, w->P, w->D, ib, nF, Dtime, (int)t);
           CK(hipGetLastError());
         }
+        if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
       }
       hipLaunchKernelGGL(k_ws_end, dim3((H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t);
       CK(hipGetLastError());
       w->rev++;
+    }
+    if (w->timing && w->P.fbii) {
+      CK(hipStreamSynchronize(w->stream));
+      for (long t = 0; t < n; t++) {
+        float ms = 0.0f;
+        CK(hipEventElapsedTime(&ms, w->ev[2 * t], w->ev[2 * t + 1]));
+        w->train_ms += ms;
+        w->train_launches++;
+      }
     }
     int rc = ws_download(w, n, cm ? cm + (size_t)done * G * 4 : NULL, mon ? mon + (size_t)done * G * 18 : NULL,
                          ion ? ion + (size_t)done * G * 3 : NULL, zcm ? zcm + (size_t)done * H : NULL);
@@ -1063,3 +1081,26 @@ extern "C" int mbt_ws_distribution(const mbt_ws *w, double *dist)
 }
 
 extern "C" long mbt_ws_turn(const mbt_ws *w) { return (long)w->rev; }
+
+extern "C" int mbt_ws_set_timing(mbt_ws *w, int on)
+{
+  CK(hipSetDevice(w->device));
+  if (on && !w->ev) {
+    w->ev = (hipEvent_t *)calloc(2 * WS_T, sizeof(hipEvent_t));
+    if (!w->ev) return MBT_ENOMEM;
+    for (int i = 0; i < 2 * WS_T; i++) CK(hipEventCreate(&w->ev[i]));
+  }
+  w->timing = on ? 1 : 0;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_train_time(const mbt_ws *w, double *ms, long *launches)
+{
+  if (!w || !ms || !launches) {
+    mbt_set_error("mbt_ws_train_time: null argument");
+    return MBT_EINVAL;
+  }
+  *ms = w->train_ms;
+  *launches = w->train_launches;
+  return MBT_OK;
+}
