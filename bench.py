@@ -6,34 +6,39 @@ macroparticles, 100 longitudinal slices per bunch with integer counts,
 broadband resonator (10 GHz, Q 1, 5 kOhm) short-range wake, resistive-wall
 long-range wake over 416 buckets x 50 past turns, 0.5 mA/bunch, quantum
 excitation off, FP64.  A step is one turn of the whole ring with every
-coordinate resident in HBM; the initial distribution (reference generator) and
-set-up are outside the timed region.
+coordinate resident in HBM; the initial distribution (the reference's
+generator, made on the device) and set-up are outside the timed region.
 
-Prints one JSON line (see the fields below).  `roofline` describes the
-per-particle pass kernel k_turn: 96 algorithmic bytes per macroparticle-turn
-(read + write of six FP64 coordinates) over its average duration measured with
-HIP events on the ring's stream.  The timed turns run the library's default
-path (each block of turns replayed as one HIP graph, instantiated by an untimed
-block of the same length first); the same number of turns is then launched
-eagerly with HIP events around every kernel class (`instrumented`,
-`kernel_ms_per_turn`, `roofline`), since the runtime does not time events
-inside a graph.  `cpu_baseline` times the
-reference's own code (oracle/_ref/refharness, built from the reference sources)
-or, when that binary is absent, the project's C restatement, on the same ring for
-a bounded number of turns.
+Prints one JSON line.  `roofline` describes the per-particle pass kernel
+k_turn: 96 algorithmic bytes per macroparticle-turn (read + write of six FP64
+coordinates) over its average duration measured with HIP events on the ring's
+stream; its `traffic` comes from a separate rocprofv3 PMC profile
+(`traffic_source`).  The timed turns run the library's default path (each block
+of turns replayed as one HIP graph, instantiated by an untimed block of the
+same length first; `graphs_built_in_timed_region` counts instantiations inside
+the timed call); the same number of turns is then launched eagerly with HIP
+events around every kernel class (`instrumented`, `kernel_ms_per_turn`,
+`roofline`), since the runtime does not time events inside a graph.
+`cpu_baseline` times the reference's own turn loop (oracle/_ref/refharness,
+built from the reference sources) on the same ring, the bunches of each turn
+spread over the host's cores (`cores`), with a one-thread figure beside it;
+without that binary, the project's C restatement on one core.
 
 `batched` is the ensemble workload: 256 independent rings (seeded draws of
 bunch count, macroparticle count, current and chromaticity; RF beam-loading
-feedback on) tracked together by the same kernels, FP64, with its own
-throughput and k_turn roofline; the work decomposition is chosen by the
-autotuner (mbt_ring_autotune) first; `batched.mixed` repeats it in mixed precision
-(FP32 wake convolutions, FP32 hi/lo MFMA resistive-wall GEMM) with its
-statistics error against the FP64 run and its ratio to the plain FP64 small
-ring.  `c1` is that small ring (64 bunches x 1,000 macroparticles, QE on) in
-FP64 and mixed precision, with the mixed run's errors and the growth rates of
-a resistive-wall instability in both.  `fbii` is the weak-strong fast beam-ion
-workload (356-bunch train in h = 416), in ion-slice interactions/s, with the
-reference's own functions timed on a 48-bunch train beside it.
+feedback on; every bunch from the reference generator) tracked together by
+the same kernels, FP64, with its own throughput, k_turn roofline, placement
+search and per-ring growth rates over further turns; `batched.mixed` repeats it
+in mixed precision (FP32 wake convolutions, FP32 hi/lo MFMA resistive-wall
+GEMM) with its errors against the FP64 run; `batched.one_ring_at_a_time_fp64`
+tracks the same rings one after another through the single-ring FP64 path,
+and `vs_one_ring_at_a_time` is the batched forms' ratio to it.  `c1` is the
+small ring (64 bunches x 1,000 macroparticles, QE on) in FP64 and mixed
+precision (the lone ring's RW sum as the FP32 GEMM), with the mixed run's
+errors and the growth rates of a resistive-wall instability in both.  `fbii`
+is the weak-strong fast beam-ion workload (356-bunch train in h = 416), in
+ion-slice interactions/s, with the ion-train kernel's FP64 roofline and the
+reference's own functions timed beside it.
 """
 import argparse
 import json

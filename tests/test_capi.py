@@ -50,3 +50,24 @@ def test_ensemble_argument_checks(tmp_path):
         assert "no ring" in str(e)
     else:
         raise AssertionError("expected failure")
+
+
+def test_ring_rejects_invalid_configurations(tmp_path):
+    """Configurations the turn cannot run are rejected with a message before any
+    device call: no macroparticles, no monitor interval, wakes without slices,
+    RF feedback without a longitudinal long-range resonator, a STRONG model."""
+    from variants import write_variant
+
+    cases = [("np0", "wakes", dict(distribution={"Np": 0}), "empty ring"),
+             ("mon0", "wakes", dict(tracking={"NrevMon": 0}), "NrevMon"),
+             ("noncell", "wakes", dict(selffield={"Ncell": 0}), "Ncell"),
+             ("strong", "wakes", dict(tracking={"BunchModel": "STRONG"}), "weak-strong")]
+    for name, variant, over, msg in cases:
+        d = str(tmp_path / name)
+        cfg = m.Config(write_variant(variant, d, **over), d + "/")
+        try:
+            m.Ring(cfg)
+        except m.MbtError as e:
+            assert msg in str(e), (name, str(e))
+        else:
+            raise AssertionError("%s: expected a rejection" % name)

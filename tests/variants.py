@@ -86,6 +86,12 @@ VARIANTS = {
                                  "ring": _ring(), "distribution": {"Np": 999}},
                        extra_sections=[BBR_L, ("longrange_resonator_1", {"plane": "LON", "fres": 352.5, "Rs": 1e6,
                                                                          "Qfactor": 3e3})]),
+    # edge: one macroparticle per bunch (the generator's offset-only case, zero
+    # rms in every plane), 20 slices, RW over two turns
+    "edge_np1": dict(sections={"selffield": {"Ncell": 20, "Nsigma": 5.0},
+                               "tracking": {"NrevTot": 20, "EnableRW_long": 1, "Nmlt": 2},
+                               "ring": _ring(), "distribution": {"Np": 1}},
+                     extra_sections=[BBR_L]),
     # wake functions read from files (construct_greensfunc_file): a longitudinal
     # file in the LON plane, a transverse one in VER and HOR, each sampled finely,
     # then coarsely (steps of about 2.5 bins, the branch that fills several bins
