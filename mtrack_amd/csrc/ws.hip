@@ -191,6 +191,8 @@ __device__ void ws_cm(const double *u, const double *p, const int *npop, int S, 
 __device__ __forceinline__ void ws_interp(const double2 *t, double aAx, double aAz, double iax, double iaz, double xx,
                                           double yy, double &Ex, double &Ey)
 {
+  // the bilinear steps as fused multiply-adds (one rounding less each)
+#pragma clang fp contract(fast)
   // Grid point (kx, kz), kx, kz in [-2, 2], is entry 5 kx + kz + 12.  The cell
   // is k1 = trunc(x / d) clamped to the grid, its partner one step further from
   // the centre in the direction of the coordinate's sign (the same point at the
@@ -605,6 +607,8 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
         const int c = c0 + k;
         double sx = 0.0, sy = 0.0;
         if (act && c < S) {
+          // the ion's kick as fused multiply-adds
+#pragma clang fp contract(fast)
           double Ex, Ey;
           ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
           const double ax = P.factF * Ex;
