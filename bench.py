@@ -411,7 +411,8 @@ def fbii(steps, warmup, cpu=True):
            "roofline": {"bound": "fp64-valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
                         "note": "k_ws_ions: %d FP64 operations per ion-slice interaction (the reference's loop, "
-                                "src/fbi_weakstrong.c:370-441,454-580), none of them fused; duration from HIP "
+                                "src/fbi_weakstrong.c:370-441,454-580), a fused multiply-add counted as two; "
+                                "duration from HIP "
                                 "events around the kernel on its stream" % FLOPS_PER_INTERACTION}}
     if cpu:
         # the reference's beam-ion turn (oracle/_ref/refwsharness), turn loop only,
