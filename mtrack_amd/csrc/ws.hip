@@ -112,13 +112,14 @@ struct WSD {
 // (lane 7 - i within each 8), 0x4E = quad_perm [2,3,0,1], 0xB1 = quad_perm
 // [1,0,3,2].  Every partner is an involution, so both lanes of a pair form the
 // same sum and a butterfly over these four steps leaves the same total in all
-// 16 lanes, as the xor butterfly does.
+// 16 lanes, as the xor butterfly does.  Every partner is a lane of the row, so
+// bound_ctrl changes no value; it spares the move of the unused old operand.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v)
 {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -139,25 +140,43 @@ __device__ __forceinline__ double wsum64(double v)
   return rowsum16(v);
 }
 
+// One halving exchange over lane distance D = 32 or 16 without selects: lanes
+// with bit D clear return a + (a of the lane D up), lanes with it set (b of the
+// lane D down) + b.  v_permlane32_swap exchanges lanes 32-63 of its first
+// operand with lanes 0-31 of its second, v_permlane16_swap the odd rows of 16
+// of the first with the even rows of the second; one per 32-bit half.
+template <int D>
+__device__ __forceinline__ double swap_sum(double a, double b)
+{
+  const unsigned long long ia = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ib = (unsigned long long)__double_as_longlong(b);
+  unsigned lo0, lo1, hi0, hi1;
+  if (D == 32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ia, (unsigned)ib, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ia >> 32), (unsigned)(ib >> 32), false, false);
+    lo0 = lo[0], lo1 = lo[1], hi0 = hi[0], hi1 = hi[1];
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ia, (unsigned)ib, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ia >> 32), (unsigned)(ib >> 32), false, false);
+    lo0 = lo[0], lo1 = lo[1], hi0 = hi[0], hi1 = hi[1];
+  }
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi0 << 32) | lo0));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi1 << 32) | lo1));
+  return x + y;
+}
+
 // Eight sums over the 64 lanes of a wave at once: three halving exchanges (xor
 // 32, 16, 8) split the eight values between lane groups, three more (xor 4, 2,
 // 1) finish each; lanes 8k .. 8k+7 return the total of v[k].  Ten exchanges
 // instead of 48 for eight separate reductions; a fixed tree, so reproducible.
 __device__ __forceinline__ double wsum64x8(const double v[8])
 {
-  const int lane = threadIdx.x & 63;
-  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+  const bool b3 = threadIdx.x & 8;
   double w[4], x[2];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const double mine = b5 ? v[4 + k] : v[k], other = b5 ? v[k] : v[4 + k];
-    w[k] = mine + __shfl_xor(other, 32, 64);
-  }
+  for (int k = 0; k < 4; k++) w[k] = swap_sum<32>(v[k], v[4 + k]);
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const double mine = b4 ? w[2 + k] : w[k], other = b4 ? w[k] : w[2 + k];
-    x[k] = mine + __shfl_xor(other, 16, 64);
-  }
+  for (int k = 0; k < 2; k++) x[k] = swap_sum<16>(w[k], w[2 + k]);
   // bit 3 by the row mirror (its partner 15 - i has bit 3 flipped), then the
   // remaining three bits within each half row
   double y = (b3 ? x[1] : x[0]) + dpp_d<0x140>(b3 ? x[0] : x[1]);
@@ -564,20 +583,117 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
   }
 }
 
+// The ion train's field table of a slice in LDS: the 5 x 5 grid with its edge
+// rows and columns repeated once more, 7 x 7 entries, entry 24 + 7 kx + kz for
+// kx, kz in -3 .. 3 = the point (clamp(kx), clamp(kz)) of the grid.  The
+// partner of the reference's cell k1 (one step further from the centre in the
+// direction of the coordinate's sign, clamped to the grid) is then the plain
+// next entry: no clamp of the partner, and rows of 7 spread neighbouring points
+// over the LDS banks.
+#define WS_QT 49 // entries per slice
+
+// ws_interp on a slice's padded table: the same cells, fractions and steps.
+__device__ __forceinline__ void ws_interp_p(const double2 *t, double aAx, double aAz, double iax, double iaz,
+                                            double xx, double yy, double &Ex, double &Ey)
+{
+  // the bilinear steps as fused multiply-adds (one rounding less each)
+#pragma clang fp contract(fast)
+  const int k1x = min(max((int)(xx * iax), -2), 2);
+  const int k1z = min(max((int)(yy * iaz), -2), 2);
+  const int dx = xx >= 0 ? 7 : -7, dz = yy >= 0 ? 1 : -1;
+  const double2 *e = t + 24 + 7 * k1x + k1z;
+  const double2 e1 = e[0], e2 = e[dx], e3 = e[dz], e4 = e[dx + dz];
+  const double sx = fabs((xx - k1x * aAx) * iax), sz = fabs((yy - k1z * aAz) * iaz);
+  const double y5 = e1.y + sx * (e2.y - e1.y);
+  const double y6 = e3.y + sx * (e4.y - e3.y);
+  const double x5 = e1.x + sx * (e2.x - e1.x);
+  const double x6 = e3.x + sx * (e4.x - e3.x);
+  Ey = y5 + sz * (y6 - y5);
+  Ex = x5 + sz * (x6 - x5);
+}
+
+// One group of four slices c0 .. c0 + 3 of a bunch for the lane's ion: the
+// kicks, then the group's eight field sums (Nri Ex, Nri Ey per slice) reduced
+// over the wave into wred.  ALL: every lane's ion is in the cloud (lanes past
+// the last ion carry Nri = 0 from a finite state, so they add exact zeros);
+// FULL: all four slices exist.
+template <bool ALL, bool FULL>
+__device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int c0, int S, bool act, double Nri,
+                                           double &x, double &xd, double &y, double &yd,
+                                           double (*wred)[WS_WG / 64], int lane, int wv)
+{
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int c = c0 + k;
+    double sx = 0.0, sy = 0.0;
+    if ((ALL || act) && (FULL || c < S)) {
+      // the ion's kick as fused multiply-adds
+#pragma clang fp contract(fast)
+      double Ex, Ey;
+      ws_interp_p(qt + c * WS_QT, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+      const double ax = P.factF * Ex;
+      xd = xd + P.deltT * ax;
+      x = x + P.deltT * xd;
+      const double ay = P.factF * Ey;
+      yd = yd + P.deltT * ay;
+      y = y + P.deltT * yd;
+      sx = Nri * Ex;
+      sy = Nri * Ey;
+    }
+    v[2 * k] = sx;
+    v[2 * k + 1] = sy;
+  }
+  const double t = wsum64x8(v);
+  const int idx = lane >> 3, c = c0 + (idx >> 1);
+  if ((lane & 7) == 0 && (FULL || c < S)) wred[2 * c + (idx & 1)][wv] = t;
+}
+
+// One bunch ib of the train for the lane's ion (drift, then every slice),
+// followed by the cloud sums Nri x, Nri y, Nri over the wave into wred.
+template <bool ALL>
+__device__ __forceinline__ void ions_bunch(const WSP &P, const WSD &D, const double2 *qt, int ib, bool act,
+                                           double Nri, double &x, double &xd, double &y, double &yd,
+                                           double (*wred)[WS_WG / 64], int lane, int wv)
+{
+  const int S = P.S;
+  if (ALL || act) {
+    const double dt = D.dtime[ib];
+    x = x + dt * xd;
+    y = y + dt * yd;
+  }
+  int c0 = 0;
+  for (; c0 + 4 <= S; c0 += 4) ions_group<ALL, true>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  if (c0 < S) ions_group<ALL, false>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  double s0 = ALL || act ? Nri * x : 0.0, s1 = ALL || act ? Nri * y : 0.0, s2 = ALL || act ? Nri : 0.0;
+  s0 = wsum64(s0);
+  s1 = wsum64(s1);
+  s2 = wsum64(s2);
+  if (lane == 0) {
+    wred[2 * S][wv] = s0;
+    wred[2 * S + 1][wv] = s1;
+    wred[2 * S + 2][wv] = s2;
+  }
+}
+
 // Ion-parallel train: lane q carries ion q = j * Nions + p (ion p made by bunch
 // j, its state after the creator's later slices) through bunches j+1 .. G-1,
 // each reached after its drift time.  Per bunch ib the block writes the 2 S
 // slice sums of Nri * E and the cloud sums Nri x, Nri y, Nri after the bunch,
-// each the fixed-order sum of its four waves' sums.
+// each the fixed-order sum of its four waves' sums.  The bunches up to the
+// block's last creator (some lanes' ions not yet made) run with per-lane
+// predicates, the rest without.  Dynamic LDS: S * WS_QT double2 (the padded
+// tables of one bunch).
 __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
 {
-  __shared__ double2 exy_s[WS_PTS];
+  extern __shared__ double2 qt_s[];
   __shared__ double wred[2 * WS_S + 3][WS_WG / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25, NQ = 2 * S + 3;
   const size_t GP = (size_t)P.G * npts;
   const long long nI = (long long)(P.G - 1) * npts;
   const long long q0 = (long long)blockIdx.x * WS_WG, q = q0 + tid;
   const int jmin = (int)(q0 / npts);
+  const int jmax = (int)(min(q0 + WS_WG - 1, nI - 1) / npts);
   const bool valid = q < nI;
   const int j = valid ? (int)(q / npts) : P.G;
   double Nri = 0.0, x = 0.0, xd = 0.0, y = 0.0, yd = 0.0;
@@ -588,54 +704,20 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
     y = D.li[3 * GP + q];
     yd = D.li[4 * GP + q];
   }
+  // threads 0 .. 4 WS_QT - 1 stage entry r of the tables of slices sl0,
+  // sl0 + 4, ...: a fixed grid point per thread
+  const int sl0 = tid / WS_QT, r = tid % WS_QT;
+  const int src = 12 + 5 * min(max(r / 7 - 3, -2), 2) + min(max(r % 7 - 3, -2), 2);
   for (int ib = jmin + 1; ib < P.G; ib++) {
     __syncthreads(); // the previous bunch's table and sums have been read
-    for (int k = tid; k < npts; k += WS_WG)
-      exy_s[k] = make_double2(D.Ex[(size_t)ib * npts + k], D.Ey[(size_t)ib * npts + k]);
+    const double *ex = D.Ex + (size_t)ib * npts + src, *ey = D.Ey + (size_t)ib * npts + src;
+    if (tid < 4 * WS_QT)
+      for (int c = sl0; c < S; c += 4) qt_s[c * WS_QT + r] = make_double2(ex[c * 25], ey[c * 25]);
     __syncthreads();
-    const bool act = j < ib;
-    if (act) {
-      const double dt = D.dtime[ib];
-      x = x + dt * xd;
-      y = y + dt * yd;
-    }
-    // four slices at a time: their eight field sums reduced together
-    for (int c0 = 0; c0 < S; c0 += 4) {
-      double v[8];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int c = c0 + k;
-        double sx = 0.0, sy = 0.0;
-        if (act && c < S) {
-          // the ion's kick as fused multiply-adds
-#pragma clang fp contract(fast)
-          double Ex, Ey;
-          ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
-          const double ax = P.factF * Ex;
-          xd = xd + P.deltT * ax;
-          x = x + P.deltT * xd;
-          const double ay = P.factF * Ey;
-          yd = yd + P.deltT * ay;
-          y = y + P.deltT * yd;
-          sx = Nri * Ex;
-          sy = Nri * Ey;
-        }
-        v[2 * k] = sx;
-        v[2 * k + 1] = sy;
-      }
-      const double t = wsum64x8(v);
-      const int idx = lane >> 3, c = c0 + (idx >> 1);
-      if ((lane & 7) == 0 && c < S) wred[2 * c + (idx & 1)][wv] = t;
-    }
-    double s0 = act ? Nri * x : 0.0, s1 = act ? Nri * y : 0.0, s2 = act ? Nri : 0.0;
-    s0 = wsum64(s0);
-    s1 = wsum64(s1);
-    s2 = wsum64(s2);
-    if (lane == 0) {
-      wred[2 * S][wv] = s0;
-      wred[2 * S + 1][wv] = s1;
-      wred[2 * S + 2][wv] = s2;
-    }
+    if (ib <= jmax)
+      ions_bunch<false>(P, D, qt_s, ib, j < ib, Nri, x, xd, y, yd, wred, lane, wv);
+    else
+      ions_bunch<true>(P, D, qt_s, ib, true, Nri, x, xd, y, yd, wred, lane, wv);
     __syncthreads();
     for (int i = tid; i < NQ; i += WS_WG) {
       double v = 0.0;
@@ -1023,7 +1105,8 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
       CK(hipGetLastError());
       if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
       if (w->P.fbii && !w->serial) {
-        hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), 0, w->stream, w->P, w->D);
+        hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * w->P.S,
+                           w->stream, w->P, w->D);
         CK(hipGetLastError());
         if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
         hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);

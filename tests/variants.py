@@ -172,12 +172,12 @@ def conf_template(text, d):
 WS_FILL = {20: [int(k * 20 / 12) for k in range(12)], 416: [int(k * 416 / 12) for k in range(12)]}
 
 
-def _ws(h, fbii, rw, nmlt=3, turns=30, np_=2000, filling="@FILL@"):
+def _ws(h, fbii, rw, nmlt=3, turns=30, np_=2000, filling="@FILL@", nslc=20):
     return dict(sections={
         "tracking": {"BunchModel": "STRONG", "EnableFBII": fbii, "NrevTot": turns, "NrevMon": 10,
                      "EnableRW_long": rw, "Nmlt": nmlt, "EnableQuantum": 0, "filling": filling},
         "ring": dict(Lc=cg.small_ring(h), Iring=100.0, **RW),
-        "distribution": {"Np": np_, "Nslc": 20}})
+        "distribution": {"Np": np_, "Nslc": nslc}})
 
 
 # name: (configuration, bunch-current mode: 1 = currents of the filling, 0 = the
@@ -187,6 +187,9 @@ WS_VARIANTS = {
     "ws_fbii_refcur": (_ws(20, 1, 1), 0),
     "ws_optics": (_ws(20, 0, 0), 1),
     "ws_h416": (_ws(416, 1, 1, nmlt=12), 1),
+    # 19 slices a side: 39 centroids (the reference bins at 0.3 sigma from
+    # -7 sigma, so it needs about 17 or more), 39 = 9 x 4 + 3 slices
+    "ws_s39": (_ws(20, 1, 1, nslc=19), 1),
     # every bucket filled (the reference program's manager addresses bucket kb
     # as rank kb + 1, so its end-to-end run needs buckets 0..Nbunch-1 filled)
     "ws_uniform": (_ws(20, 1, 1, filling="UNIFORM"), 0),
