@@ -91,7 +91,7 @@ struct WSD {
   const int *bucket, *fill; // [G], [h]
   const double *Ibunch, *nelec, *convion; // [h] mA, [G], [G]
   double *hist;     // [4][h][WS_HIST] x x' z z' (the manager's history)
-  double *Ex, *Ey;  // [G][S*25]
+  double2 *Exy;     // [G][S*25] (Ex, Ey) of every grid point
   double *li;       // [5][G][S*25] local ions: Nri x x' y y'
   double *cl;       // [5][cap] ion cloud
   long long cap;
@@ -330,8 +330,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev
     const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
     const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
     exy_s[q] = make_double2(Ex, Ey);
-    D.Ex[(size_t)b * npts + q] = Ex;
-    D.Ey[(size_t)b * npts + q] = Ey;
+    D.Exy[(size_t)b * npts + q] = make_double2(Ex, Ey);
   }
   __syncthreads();
   // ---- the ions this bunch creates: one per local grid point, kicked by the
@@ -465,7 +464,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
   __shared__ int last;
   const int tid = threadIdx.x, S = P.S, npts = S * 25;
   const size_t GP = (size_t)P.G * npts;
-  for (int q = tid; q < npts; q += WS_WG) exy_s[q] = make_double2(D.Ex[(size_t)ib * npts + q], D.Ey[(size_t)ib * npts + q]);
+  for (int q = tid; q < npts; q += WS_WG) exy_s[q] = D.Exy[(size_t)ib * npts + q];
   __syncthreads();
   const long long q = (long long)blockIdx.x * WS_WG + tid;
   const bool act = q < nF;
@@ -612,6 +611,25 @@ __device__ __forceinline__ void ws_interp_p(const double2 *t, double aAx, double
   Ex = x5 + sz * (x6 - x5);
 }
 
+// One slice c of a bunch for the lane's ion: the field at the ion, the ion's
+// kick, and the ion's share Nri E of the slice's kick.
+__device__ __forceinline__ void ion_slice(const WSP &P, const double2 *qt, int c, double Nri, double &x, double &xd,
+                                          double &y, double &yd, double &sx, double &sy)
+{
+  // the ion's kick as fused multiply-adds
+#pragma clang fp contract(fast)
+  double Ex, Ey;
+  ws_interp_p(qt + c * WS_QT, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+  const double ax = P.factF * Ex;
+  xd = xd + P.deltT * ax;
+  x = x + P.deltT * xd;
+  const double ay = P.factF * Ey;
+  yd = yd + P.deltT * ay;
+  y = y + P.deltT * yd;
+  sx = Nri * Ex;
+  sy = Nri * Ey;
+}
+
 // One group of four slices c0 .. c0 + 3 of a bunch for the lane's ion: the
 // kicks, then the group's eight field sums (Nri Ex, Nri Ey per slice) reduced
 // over the wave into wred.  ALL: every lane's ion is in the cloud (lanes past
@@ -627,26 +645,42 @@ __device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int 
   for (int k = 0; k < 4; k++) {
     const int c = c0 + k;
     double sx = 0.0, sy = 0.0;
-    if ((ALL || act) && (FULL || c < S)) {
-      // the ion's kick as fused multiply-adds
-#pragma clang fp contract(fast)
-      double Ex, Ey;
-      ws_interp_p(qt + c * WS_QT, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
-      const double ax = P.factF * Ex;
-      xd = xd + P.deltT * ax;
-      x = x + P.deltT * xd;
-      const double ay = P.factF * Ey;
-      yd = yd + P.deltT * ay;
-      y = y + P.deltT * yd;
-      sx = Nri * Ex;
-      sy = Nri * Ey;
-    }
+    if ((ALL || act) && (FULL || c < S)) ion_slice(P, qt, c, Nri, x, xd, y, yd, sx, sy);
     v[2 * k] = sx;
     v[2 * k + 1] = sy;
   }
   const double t = wsum64x8(v);
   const int idx = lane >> 3, c = c0 + (idx >> 1);
   if ((lane & 7) == 0 && (FULL || c < S)) wred[2 * c + (idx & 1)][wv] = t;
+}
+
+// The last r = 1 or 2 slices c0 .. c0 + r - 1 of a bunch and, after them, the
+// cloud sums Nri x, Nri y, Nri: one eight-value reduction for both.
+template <bool ALL>
+__device__ __forceinline__ void ions_tail(const WSP &P, const double2 *qt, int c0, int r, int S, bool act,
+                                          double Nri, double &x, double &xd, double &y, double &yd,
+                                          double (*wred)[WS_WG / 64], int lane, int wv)
+{
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    double sx = 0.0, sy = 0.0;
+    if ((ALL || act) && k < r) ion_slice(P, qt, c0 + k, Nri, x, xd, y, yd, sx, sy);
+    v[2 * k] = sx;
+    v[2 * k + 1] = sy;
+  }
+  v[4] = ALL || act ? Nri * x : 0.0;
+  v[5] = ALL || act ? Nri * y : 0.0;
+  v[6] = ALL || act ? Nri : 0.0;
+  v[7] = 0.0;
+  const double t = wsum64x8(v);
+  const int m = lane >> 3;
+  if ((lane & 7) == 0) {
+    if (m < 2 * r)
+      wred[2 * c0 + m][wv] = t;
+    else if (m >= 4 && m < 7)
+      wred[2 * S + m - 4][wv] = t;
+  }
 }
 
 // One bunch ib of the train for the lane's ion (drift, then every slice),
@@ -664,7 +698,12 @@ __device__ __forceinline__ void ions_bunch(const WSP &P, const WSD &D, const dou
   }
   int c0 = 0;
   for (; c0 + 4 <= S; c0 += 4) ions_group<ALL, true>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
-  if (c0 < S) ions_group<ALL, false>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  const int r = S - c0;
+  if (r == 1 || r == 2) {
+    ions_tail<ALL>(P, qt, c0, r, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+    return;
+  }
+  if (r) ions_group<ALL, false>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
   double s0 = ALL || act ? Nri * x : 0.0, s1 = ALL || act ? Nri * y : 0.0, s2 = ALL || act ? Nri : 0.0;
   s0 = wsum64(s0);
   s1 = wsum64(s1);
@@ -710,9 +749,9 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
   const int src = 12 + 5 * min(max(r / 7 - 3, -2), 2) + min(max(r % 7 - 3, -2), 2);
   for (int ib = jmin + 1; ib < P.G; ib++) {
     __syncthreads(); // the previous bunch's table and sums have been read
-    const double *ex = D.Ex + (size_t)ib * npts + src, *ey = D.Ey + (size_t)ib * npts + src;
+    const double2 *exy = D.Exy + (size_t)ib * npts + src;
     if (tid < 4 * WS_QT)
-      for (int c = sl0; c < S; c += 4) qt_s[c * WS_QT + r] = make_double2(ex[c * 25], ey[c * 25]);
+      for (int c = sl0; c < S; c += 4) qt_s[c * WS_QT + r] = exy[c * 25];
     __syncthreads();
     if (ib <= jmax)
       ions_bunch<false>(P, D, qt_s, ib, j < ib, Nri, x, xd, y, yd, wred, lane, wv);
@@ -824,7 +863,7 @@ extern "C" void mbt_ws_destroy(mbt_ws *w)
 {
   if (!w) return;
   (void)hipSetDevice(w->device);
-  void *p[] = {w->D.sl,  w->D.hist, w->D.Ex,   w->D.Ey,  w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
+  void *p[] = {w->D.sl,  w->D.hist, w->D.Exy, w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
                w->D.done, w->D.cm,  w->D.mon,  w->D.ion, w->D.zcm,    w->d_bucket, w->d_fill,    w->d_npop,
                w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff};
   for (size_t i = 0; i < sizeof p / sizeof p[0]; i++)
@@ -994,8 +1033,7 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
     WA(w->d_nelec, G);
     WA(w->d_convion, G);
     WA(w->D.hist, (size_t)4 * H * WS_HIST);
-    WA(w->D.Ex, (size_t)G * npts);
-    WA(w->D.Ey, (size_t)G * npts);
+    WA(w->D.Exy, (size_t)G * npts);
     WA(w->D.li, (size_t)5 * G * npts);
     // train: drift times, then the ion-parallel form's partials (one block
     // row per bunch for every block holding an ion made ahead of it), or the
