@@ -51,6 +51,7 @@
 #define WS_MSG 5000   // doubles of the history message (src/manager_weakstrong.c:170-173)
 #define WS_T 256      // turns of outputs per device buffer block
 #define WS_WG 256
+#define KICK_WG 1024 // k_ws_kick
 // device memory the ion-parallel train may use for its per-bunch partials
 #define WS_TRAIN_PARTIAL_MAX (8LL << 30)
 
@@ -99,7 +100,7 @@ struct WSD {
   double *psum;     // [blocks][3]
   unsigned *done;
   const double *dtime;    // [G] drift time from the previous bunch
-  double *tpart;          // ion-parallel train: per bunch ib, [blocks with an ion ahead of ib][2 S + 3]
+  double *tpart;          // ion-parallel train: per bunch ib, [2 S + 3][blocks with an ion ahead of ib]
   const long long *toff;  // [G] offset of bunch ib's partials in tpart
   int nblk_ions;          // blocks of k_ws_ions
   double *cm, *mon, *ion, *zcm; // output blocks [WS_T][G][4|18|3], [WS_T][h]
@@ -758,10 +759,12 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
     else
       ions_bunch<true>(P, D, qt_s, ib, true, Nri, x, xd, y, yd, wred, lane, wv);
     __syncthreads();
+    // quantity-major, so that k_ws_kick reads each quantity's partials contiguously
+    const long long nbi = min(((long long)ib * npts + WS_WG - 1) / WS_WG, (long long)D.nblk_ions);
     for (int i = tid; i < NQ; i += WS_WG) {
       double v = 0.0;
       for (int w = 0; w < WS_WG / 64; w++) v += wred[i][w];
-      D.tpart[D.toff[ib] + (long long)blockIdx.x * NQ + i] = v;
+      D.tpart[D.toff[ib] + i * nbi + blockIdx.x] = v;
     }
   }
 }
@@ -769,41 +772,59 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
 // Per bunch: the blocks' partials summed in a fixed order (wave i of the
 // quantities, lanes over blocks), the slice kicks of the ion field, and the
 // cloud sums after the bunch with its own ions appended.
-__global__ __launch_bounds__(WS_WG) void k_ws_kick(WSP P, WSD D, int slot)
+__global__ __launch_bounds__(KICK_WG) void k_ws_kick(WSP P, WSD D, int slot)
 {
   __shared__ double tot[2 * WS_S + 3];
   __shared__ double red3[WS_WG / 64][3];
   const int ib = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25;
   const int NQ = 2 * S + 3;
   const size_t GP = (size_t)P.G * npts;
-  long long nb = ((long long)ib * npts + WS_WG - 1) / WS_WG;
+  long long nb = ((long long)ib * npts + WS_WG - 1) / WS_WG; // blocks of k_ws_ions ahead of ib
   if (nb > D.nblk_ions) nb = D.nblk_ions;
-  for (int i = wv; i < NQ; i += WS_WG / 64) {
-    double v = 0.0;
-    for (long long k = lane; k < nb; k += 64) v += D.tpart[D.toff[ib] + k * NQ + i];
-    v = wsum64(v);
-    if (lane == 0) tot[i] = v;
+  // eight quantities per wave and four blocks per lane unrolled (up to 32
+  // loads in flight per lane, 16 waves: the last bunches of a long train sum
+  // about a megabyte of partials each); each quantity's order is fixed: lane
+  // l adds blocks l, l + 64, ... in turn, then the wave sum
+  const double *tp = D.tpart + D.toff[ib];
+  for (int i0 = 8 * wv; i0 < NQ; i0 += 8 * (KICK_WG / 64)) {
+    double v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = 0.0;
+#pragma unroll 4
+    for (long long k = lane; k < nb; k += 64) {
+#pragma unroll
+      for (int m = 0; m < 8; m++)
+        if (i0 + m < NQ) v[m] += tp[(i0 + m) * nb + k];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const double t = wsum64(v[m]);
+      if (lane == 0 && i0 + m < NQ) tot[i0 + m] = t;
+    }
   }
-  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-  for (int p = tid; p < npts; p += WS_WG) {
-    const size_t o = (size_t)ib * npts + p;
-    t0 += D.li[o] * D.li[GP + o];
-    t1 += D.li[o] * D.li[3 * GP + o];
-    t2 += D.li[o];
-  }
-  t0 = wsum64(t0);
-  t1 = wsum64(t1);
-  t2 = wsum64(t2);
-  if (lane == 0) {
-    red3[wv][0] = t0;
-    red3[wv][1] = t1;
-    red3[wv][2] = t2;
+  // the bunch's own ions: the first WS_WG threads, four waves' sums in order
+  if (tid < WS_WG) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int p = tid; p < npts; p += WS_WG) {
+      const size_t o = (size_t)ib * npts + p;
+      t0 += D.li[o] * D.li[GP + o];
+      t1 += D.li[o] * D.li[3 * GP + o];
+      t2 += D.li[o];
+    }
+    t0 = wsum64(t0);
+    t1 = wsum64(t1);
+    t2 = wsum64(t2);
+    if (lane == 0) {
+      red3[wv][0] = t0;
+      red3[wv][1] = t1;
+      red3[wv][2] = t2;
+    }
   }
   __syncthreads();
   if (ib > 0) {
     const double nel = D.nelec[ib];
     double *xp = D.sl + (size_t)ib * 6 * S + 3 * S, *zp = xp + 2 * S;
-    for (int c = tid; c < S; c += WS_WG) {
+    for (int c = tid; c < S; c += KICK_WG) {
       const double dQ = nel * D.npop[c] * K_E;
       const double fjp = dQ > 0.0 ? P.factjp0 / dQ : 0.0;
       const double sx = fjp * tot[2 * c], sy = fjp * tot[2 * c + 1];
@@ -1147,7 +1168,7 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
                            w->stream, w->P, w->D);
         CK(hipGetLastError());
         if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
-        hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
+        hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(KICK_WG), 0, w->stream, w->P, w->D, (int)t);
         CK(hipGetLastError());
       } else if (w->P.fbii) {
         hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
