@@ -486,7 +486,7 @@ def main():
             out["batched"] = batched(a.batched_steps, 3, a.batched_rings, a.batched_seed, a.batched_chunk,
                                      a.batched_mixed, a.batched_growth_turns)[0]
         if not a.no_fbii:
-            out["fbii"] = fbii(a.fbii_steps, 2, cpu=False)
+            out["fbii"] = fbii(a.fbii_steps, 5, cpu=False)
         print(json.dumps(out))
         return 0
     import mtrack_amd as m
@@ -577,7 +577,7 @@ def main():
             out["batched"] = {"value": None, "error": str(e)[:300]}
     if not a.no_fbii:
         try:
-            out["fbii"] = fbii(a.fbii_steps, 2, cpu=not a.no_cpu_baseline)
+            out["fbii"] = fbii(a.fbii_steps, 5, cpu=not a.no_cpu_baseline)
         except Exception as e:
             out["fbii"] = {"value": None, "error": str(e)[:300]}
     if not a.no_cpu_baseline:
