@@ -233,6 +233,17 @@ int mbt_ring_prepare(mbt_ring *r, double *init_stats);
  * (written on monitor turns only), rf: nturns*2 (phai0, Vrf0 used by the turn's map).
  * Any output pointer may be NULL. */
 int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *ampinv, double *rf);
+/* mbt_ring_track split in two, so that a caller can do its per-turn host work
+ * (the reference manager's files) while the next turns run: _async enqueues
+ * nturns turns (1 .. mbt_ring_async_max(r), one block of the device's
+ * statistics ring) and returns; _wait waits for them and copies their outputs
+ * as mbt_ring_track does.  One block at a time: other tracking calls on the
+ * ring fail with MBT_EINVAL while one is pending.  No counterpart in the
+ * reference, whose manager receives each turn's statistics over MPI
+ * (src/manager_weakweak.c:179-299). */
+long mbt_ring_async_max(const mbt_ring *r);
+int mbt_ring_track_async(mbt_ring *r, long nturns);
+int mbt_ring_track_wait(mbt_ring *r, double *stats, double *ampinv, double *rf);
 /* Statistics after the last pending RW kick (src/worker_weakweak.c:397-405). */
 int mbt_ring_final_stats(mbt_ring *r, double *stats);
 /* Selffield histogram of the last completed turn (clamped rule,

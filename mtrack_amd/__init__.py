@@ -59,6 +59,11 @@ def lib():
         L.mbt_ring_get_bunch.argtypes = [vp, ip, dp]
         L.mbt_ring_prepare.argtypes = [vp, dp]
         L.mbt_ring_track.argtypes = [vp, ctypes.c_long, dp, dp, dp]
+        L.mbt_ring_async_max.argtypes = [vp]
+        L.mbt_ring_async_max.restype = ctypes.c_long
+        L.mbt_ring_track_async.argtypes = [vp, ctypes.c_long]
+        L.mbt_ring_track_wait.argtypes = [vp, dp, dp, dp]
+        L.mbt_ring_potentials.argtypes = [vp, ip, dp, dp]
         L.mbt_ring_final_stats.argtypes = [vp, dp]
         L.mbt_ring_histograms.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
         L.mbt_ring_kernel_times.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
@@ -203,6 +208,33 @@ class Ring:
         rf = np.zeros((nturns, 2)) if want_rf else None
         _check(lib().mbt_ring_track(self.ptr, nturns, dptr(st), dptr(am), dptr(rf)))
         return st, am, rf
+
+    def async_max(self):
+        return lib().mbt_ring_async_max(self.ptr)
+
+    def track_async(self, nturns):
+        """Enqueues nturns turns (at most async_max()) without waiting."""
+        _check(lib().mbt_ring_track_async(self.ptr, nturns))
+        self._async_n = nturns
+
+    def track_wait(self, want_stats=True, want_amp=True, want_rf=True):
+        """Waits for the turns of track_async; returns their outputs as track does."""
+        n = getattr(self, "_async_n", 0)
+        st = np.zeros((n, self.Nb, NSTAT)) if want_stats else None
+        am = np.zeros((n, self.Nb, NAMP)) if want_amp else None
+        rf = np.zeros((n, 2)) if want_rf else None
+        _check(lib().mbt_ring_track_wait(self.ptr, dptr(st), dptr(am), dptr(rf)))
+        self._async_n = 0
+        return st, am, rf
+
+    def potentials(self, ib):
+        """The tables of bunch ordinal ib the next turn's kick uses: 7 rows of Nc
+        (wake L, LR wake, counts, wake V, dipoles V, wake H, dipoles H) and the
+        previous phase, voltage and scan value."""
+        out = np.zeros((7, max(self.Nc, 1)))
+        prev = np.zeros(3)
+        _check(lib().mbt_ring_potentials(self.ptr, ib, dptr(out), dptr(prev)))
+        return out, prev
 
     def final_stats(self):
         st = np.zeros((self.Nb, NSTAT))

@@ -45,6 +45,17 @@ static FILE *wopen(const mbt_config *c, const char *name, const char *mode)
 }
 
 /* weak_writeout_bunch_distribution, src/bunch_weak.c:295-332 */
+/* turns per call of the tracking loop (the library's statistics block) */
+#define DRV_BLOCK 256
+
+/* whether the potentials files get a block at the end of turn t (the tables of
+   the output turn before its kick, src/transform_weak.c:735-768) */
+static int pot_due(const mbt_config *c, int nout, long t)
+{
+  return nout > 0 && c->Ncell > 0 && (t + 1) % c->NrevMon == 0 &&
+         t + 1 >= c->NrevPotentialsOut + (t / c->NrevScan) * c->NrevScan;
+}
+
 static int write_distribution(const mbt_config *c, int plane, const double *soa6, int Np)
 {
   char name[3 * MBT_PATH];
@@ -92,8 +103,14 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
   mbt_ring *r = NULL;
   int rc = mbt_ring_create(c, device, 0, &r);
   if (rc) return rc;
-  double *st = malloc(sizeof(double) * 12 * Nb * (size_t)(NrevMon > 0 ? NrevMon : 1));
-  double *am = calloc(6 * (size_t)Nb * (NrevMon > 0 ? NrevMon : 1), sizeof(double));
+  /* turns per library call: whole monitor blocks, up to the library's block of
+     DRV_BLOCK turns of statistics (one replayed graph per call) */
+  const long blk = NrevMon > 0 && NrevMon < DRV_BLOCK ? (DRV_BLOCK / NrevMon) * NrevMon : (NrevMon > 0 ? NrevMon : 1);
+  /* a block can hold one more turn: the previous block's potentials turn */
+  double *st = malloc(sizeof(double) * 12 * Nb * (size_t)(blk + 1));
+  double *st2 = malloc(sizeof(double) * 12 * Nb * (size_t)(blk + 1));
+  double *am = calloc(6 * (size_t)Nb * (blk + 1), sizeof(double));
+  double *am2 = calloc(6 * (size_t)Nb * (blk + 1), sizeof(double));
   double *init = malloc(sizeof(double) * 12 * Nb);
   rc = mbt_ring_prepare(r, init);
   if (rc) goto done;
@@ -178,56 +195,45 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
   printf("Starting multi bunch tracking...\n");
   fflush(stdout);
   long Nstat = 0;
-  long rev = 0;
-  while (rev < NrevTot) {
-    /* track up to the turn before the next output or block boundary */
-    long n = NrevMon - (rev % NrevMon);
-    if (rev + n > NrevTot) n = NrevTot - rev;
-    long pot_turn = rev + n - 1; /* (rev+1) % NrevMon == 0 at the block end */
-    int want_pot = 0;
-    if (nout > 0 && Nc > 0 && (pot_turn + 1) % NrevMon == 0 &&
-        pot_turn + 1 >= c->NrevPotentialsOut + (pot_turn / NrevScan) * NrevScan)
-      want_pot = 1;
-    long n1 = want_pot ? n - 1 : n;
-    if (n1 > 0) {
-      rc = mbt_ring_track(r, n1, st, am, NULL);
-      if (rc) goto done;
+  long rev = 0;  /* turns the per-turn bookkeeping has processed */
+  long trev = 0; /* turns planned: the start of the next block of whole monitor blocks */
+  const long amax = mbt_ring_async_max(r);
+  double *stb[2] = {st, st2}, *amb[2] = {am, am2};
+  long pn = 0;    /* turns of the previous block, waiting for the bookkeeping */
+  long carry = 0; /* the previous block's potentials turn, tracked at the head of the next block */
+  int pb = 0, cb = 0;
+  while (trev < NrevTot || carry > 0 || pn > 0) {
+    long n = 0, n1 = carry;
+    int want_pot = 0, pending = 0;
+    if (trev < NrevTot) {
+      /* the next block: up to the next monitor boundary, then whole monitor
+         blocks more while no potentials output falls due at a block's end;
+         the turn of a due output is left for the head of the block after,
+         so that the output's tables are read between the two */
+      n = NrevMon - (trev % NrevMon);
+      if (trev + n > NrevTot) n = NrevTot - trev;
+      while (carry + n + NrevMon <= amax && trev + n + NrevMon <= NrevTot && !pot_due(c, nout, trev + n - 1))
+        n += NrevMon;
+      want_pot = pot_due(c, nout, trev + n - 1);
+      n1 = carry + (want_pot ? n - 1 : n);
     }
-    if (want_pot) {
-      /* src/transform_weak.c:735-768: tables of the output turn, before its kick */
-      for (int i = 0; i < nout; i++) {
-        if (!fpot[i]) continue;
-        int k = outb[i], kb = c->bucket[k];
-        double extra[3];
-        if ((rc = mbt_ring_potentials(r, k, pot, extra))) goto done;
-        double eps_const = extra[1] / (c->E0 * FKILO);
-        double FacI = c->EnableIdealHC == 0 ? 0. : eps_const * c->HC_k;
-        for (int ic = 0; ic < Nc; ic++) {
-          double taucell = -(-c->Nsigma + c->dT * (ic - 0.5)) * c->sigma_tau;
-          double rftest = eps_const * sin(c->wrf * taucell + extra[0]) - c->Urad;
-          double active = 0.0;
-          for (int a = 0; a < c->n_ahc; a++)
-            active += c->ahc[a].Vpeak / (c->E0 * MBT_FGIGA) *
-                      sin(c->wrf * c->ahc[a].nHC * taucell + c->ahc[a].phi_aHC + c->ahc[a].nHC * kb * 2 * MBT_PI);
-          double ideal = FacI * sin(c->m_aHC * c->wrf * taucell + c->m_aHC * c->phi_n);
-          fprintf(fpot[i], "\n %ld   %e   %e   %e    %e   %e   %e   %e   %e", pot_turn + 1, extra[2], taucell,
-                  pot[ic], pot[Nc + ic], rftest, active, ideal, pot[2 * Nc + ic]);
-          if (planes[2] > 0) fprintf(fpot[i], "   %e   %e", pot[3 * Nc + ic], pot[4 * Nc + ic]);
-          if (planes[1] > 0) fprintf(fpot[i], "   %e   %e", pot[5 * Nc + ic], pot[6 * Nc + ic]);
-        }
-        fprintf(fpot[i], "\n");
-      }
-      rc = mbt_ring_track(r, 1, st + (size_t)12 * Nb * (n - 1), am + (size_t)6 * Nb * (n - 1), NULL);
-      if (rc) goto done;
+    if (n1 > 0 && n1 <= amax) {
+      /* enqueued now, collected after the previous block's bookkeeping */
+      if ((rc = mbt_ring_track_async(r, n1))) goto done;
+      pending = 1;
+    } else if (n1 > 0 && (rc = mbt_ring_track(r, n1, stb[cb], amb[cb], NULL))) {
+      goto done;
     }
-    /* per-turn bookkeeping of the manager and the kb_out workers */
-    for (long t = 0; t < n; t++, rev++) {
+    /* per-turn bookkeeping of the manager and the kb_out workers for the
+       previous block, while the next one runs */
+    const double *st_p = stb[pb], *am_p = amb[pb];
+    for (long t = 0; t < pn; t++, rev++) {
       if (rev % 5000 == 0) {
         if (rev + 4999 >= NrevTot) printf("Transformation for turn %3ld to %3ld", rev, NrevTot - 1);
         else printf("Transformation for turn %3ld to %3ld", rev, rev + 4999);
         fflush(stdout);
       }
-      const double *s = st + (size_t)12 * Nb * t;
+      const double *s = st_p + (size_t)12 * Nb * t;
       for (int i = 0; i < nout; i++) {
         mbt_accum_add(&wacc[i], s + 12 * outb[i]);
         wN[i]++;
@@ -244,8 +250,8 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
           size_t idx = (size_t)irevmon * Nb + k;
           if (idx < hist_n) {
             for (int p = 0; p < 3; p++) {
-              H_amp[idx].v[p] = am[(size_t)6 * (Nb * t + k) + p];
-              H_ampcm[idx].v[p] = am[(size_t)6 * (Nb * t + k) + 3 + p];
+              H_amp[idx].v[p] = am_p[(size_t)6 * (Nb * t + k) + p];
+              H_ampcm[idx].v[p] = am_p[(size_t)6 * (Nb * t + k) + 3 + p];
             }
             put_vec(&H_cm[idx], s + 12 * k, 0);
           }
@@ -296,6 +302,41 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
         printf(" done (at time: %d:%d:%d)\n", tn->tm_hour, tn->tm_min, tn->tm_sec);
       }
     }
+    pn = 0;
+    if (n1 > 0) {
+      if (pending && (rc = mbt_ring_track_wait(r, stb[cb], amb[cb], NULL))) goto done;
+      if (want_pot) {
+        const long pot_turn = trev + n - 1;
+        /* src/transform_weak.c:735-768: tables of the output turn, before its kick */
+        for (int i = 0; i < nout; i++) {
+          if (!fpot[i]) continue;
+          int k = outb[i], kb = c->bucket[k];
+          double extra[3];
+          if ((rc = mbt_ring_potentials(r, k, pot, extra))) goto done;
+          double eps_const = extra[1] / (c->E0 * FKILO);
+          double FacI = c->EnableIdealHC == 0 ? 0. : eps_const * c->HC_k;
+          for (int ic = 0; ic < Nc; ic++) {
+            double taucell = -(-c->Nsigma + c->dT * (ic - 0.5)) * c->sigma_tau;
+            double rftest = eps_const * sin(c->wrf * taucell + extra[0]) - c->Urad;
+            double active = 0.0;
+            for (int a = 0; a < c->n_ahc; a++)
+              active += c->ahc[a].Vpeak / (c->E0 * MBT_FGIGA) *
+                        sin(c->wrf * c->ahc[a].nHC * taucell + c->ahc[a].phi_aHC + c->ahc[a].nHC * kb * 2 * MBT_PI);
+            double ideal = FacI * sin(c->m_aHC * c->wrf * taucell + c->m_aHC * c->phi_n);
+            fprintf(fpot[i], "\n %ld   %e   %e   %e    %e   %e   %e   %e   %e", pot_turn + 1, extra[2], taucell,
+                    pot[ic], pot[Nc + ic], rftest, active, ideal, pot[2 * Nc + ic]);
+            if (planes[2] > 0) fprintf(fpot[i], "   %e   %e", pot[3 * Nc + ic], pot[4 * Nc + ic]);
+            if (planes[1] > 0) fprintf(fpot[i], "   %e   %e", pot[5 * Nc + ic], pot[6 * Nc + ic]);
+          }
+          fprintf(fpot[i], "\n");
+        }
+      }
+      pn = n1;
+      pb = cb;
+      cb ^= 1;
+    }
+    carry = want_pot ? 1 : 0;
+    trev += n;
   }
 
   /* ---- end of run */
@@ -421,7 +462,9 @@ int mbt_manager_weakweak(const mbt_config *c, int device)
   free(outb);
 done:
   free(st);
+  free(st2);
   free(am);
+  free(am2);
   free(init);
   mbt_ring_destroy(r);
   return rc;

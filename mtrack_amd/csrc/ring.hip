@@ -1751,11 +1751,26 @@ struct mbt_ring {
   long long rw_total, st_total, fbh_total;
   size_t lds_turn, lds_red;
   long long rev_host;
+  double *pot_host;    // pinned staging of mbt_ring_potentials: 7 Nc doubles, then a TurnState
+  long async_n;        // turns enqueued by mbt_ring_track_async, not yet collected (0: none)
+  long long async_rev0; // their first turn
   hipEvent_t *evpool; // 6 per turn of a stats chunk when timing
   double kt_ms[4];
   long kt_launches[4];
   int timing;
 };
+
+// Calls that track or change the ring's state refuse to run while an
+// asynchronous block is pending (its statistics would be overwritten).
+static int no_pending(const mbt_ring *r)
+{
+  if (r->async_n) {
+    mbt_set_error("%ld turns enqueued by mbt_ring_track_async are pending: collect them with mbt_ring_track_wait",
+                  r->async_n);
+    return MBT_EINVAL;
+  }
+  return MBT_OK;
+}
 
 static size_t turn_lds(int Nc)
 {
@@ -1901,6 +1916,8 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
 {
   if (!r) return;
   (void)hipSetDevice(r->device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream); // turns of mbt_ring_track_async may still run
+  if (r->pot_host) (void)hipHostFree(r->pot_host);
   void *ptrs[] = {r->C.tau, r->C.del, r->C.x, r->C.xp, r->C.z, r->C.zp, r->wg_first, r->wg_end, r->bunch_first,
                   r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
@@ -2527,6 +2544,12 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   DA(r->fnp_out, (size_t)G * NcA);
   DA(r->dipV_out, (size_t)G * NcA);
   DA(r->dipH_out, (size_t)G * NcA);
+  if (hipHostMalloc((void **)&r->pot_host, sizeof(double) * 7 * NcA + sizeof(TurnState)) != hipSuccess) {
+    r->pot_host = NULL;
+    mbt_set_error("hipHostMalloc of the potentials staging buffer failed");
+    mbt_ring_destroy(r);
+    return MBT_ENOMEM;
+  }
   DA(r->trash_tot, (size_t)G * 2);
   DA(r->trash_pend, (size_t)G * 2);
   DA(r->ts, R);
@@ -2953,6 +2976,10 @@ static int download_slots(mbt_ring *r, long long rev0, long n, double *stats, do
 
 extern "C" int mbt_ring_prepare(mbt_ring *r, double *init_stats)
 {
+  {
+    const int prc = no_pending(r);
+    if (prc) return prc;
+  }
   CK(hipSetDevice(r->device));
   (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   int rc;
@@ -3112,33 +3139,90 @@ extern "C" int mbt_ring_graph_builds(const mbt_ring *r, long *count)
   return MBT_OK;
 }
 
-extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *ampinv, double *rf)
+// Turns per block of a tracking call: the statistics of a block fit the device
+// ring buffer; a multiple of NrevMon where possible, so that every full block
+// replays the same graph.
+static long track_block(const mbt_ring *r)
 {
+  long block = STATS_TURNS;
+  if (r->nrev_mon <= STATS_TURNS) block -= STATS_TURNS % r->nrev_mon;
+  return block;
+}
+
+// Enqueues one block of turns (graph replay, or eager launches without
+// graphs) and returns without waiting.
+static int enqueue_block(mbt_ring *r, long n)
+{
+  int rc;
+  if (!r->use_graphs) {
+    for (long t = 0; t < n; t++)
+      if ((rc = enqueue_turn(r, NULL))) return rc;
+  } else {
+    hipGraphExec_t exec;
+    if ((rc = turn_graph(r, n, &exec))) return rc;
+    CK(hipGraphLaunch(exec, r->stream));
+    r->rev_host += n;
+  }
+  return MBT_OK;
+}
+
+extern "C" long mbt_ring_async_max(const mbt_ring *r) { return track_block(r); }
+
+extern "C" int mbt_ring_track_async(mbt_ring *r, long nturns)
+{
+  int rc;
+  if ((rc = no_pending(r))) return rc;
+  if (nturns < 1 || nturns > track_block(r)) {
+    mbt_set_error("mbt_ring_track_async: 1 .. %ld turns per call", track_block(r));
+    return MBT_EINVAL;
+  }
+  if (r->timing) {
+    mbt_set_error("mbt_ring_track_async: not with kernel timing on");
+    return MBT_EINVAL;
+  }
   CK(hipSetDevice(r->device));
   (void)hipGetLastError(); // launch checks below must not report an earlier call's error
+  const long long rev0 = r->rev_host;
+  if ((rc = enqueue_block(r, nturns))) return rc;
+  r->async_rev0 = rev0;
+  r->async_n = nturns;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ring_track_wait(mbt_ring *r, double *stats, double *ampinv, double *rf)
+{
+  if (!r->async_n) {
+    mbt_set_error("mbt_ring_track_wait: no turns pending");
+    return MBT_EINVAL;
+  }
+  CK(hipSetDevice(r->device));
+  const long n = r->async_n;
+  r->async_n = 0;
+  return download_slots(r, r->async_rev0, n, stats, ampinv, rf);
+}
+
+extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *ampinv, double *rf)
+{
   int rc;
+  if ((rc = no_pending(r))) return rc;
+  CK(hipSetDevice(r->device));
+  (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   for (int i = 0; i < 4; i++) {
     r->kt_ms[i] = 0;
     r->kt_launches[i] = 0;
   }
-  // blocks of turns whose statistics fit the device ring buffer; a multiple of
-  // NrevMon where possible, so that every full block replays the same graph
-  long block = STATS_TURNS;
-  if (r->nrev_mon <= STATS_TURNS) block -= STATS_TURNS % r->nrev_mon;
+  const long block = track_block(r);
   long done = 0;
   while (done < nturns) {
     long n = nturns - done;
     if (n > block) n = block;
     long long rev0 = r->rev_host;
     const bool timed = r->timing != 0;
-    if (timed || !r->use_graphs) {
+    if (timed) {
       for (long t = 0; t < n; t++)
-        if ((rc = enqueue_turn(r, timed ? r->evpool + 6 * t : NULL))) return rc;
-    } else {
-      hipGraphExec_t exec;
-      if ((rc = turn_graph(r, n, &exec))) return rc;
-      CK(hipGraphLaunch(exec, r->stream));
-      r->rev_host += n;
+        if ((rc = enqueue_turn(r, r->evpool + 6 * t))) return rc;
+    } else if ((rc = enqueue_block(r, n))) {
+      return rc;
     }
     if (timed) {
       CK(hipStreamSynchronize(r->stream));
@@ -3167,6 +3251,10 @@ extern "C" int mbt_ring_track(mbt_ring *r, long nturns, double *stats, double *a
 
 extern "C" int mbt_ring_final_stats(mbt_ring *r, double *stats)
 {
+  {
+    const int prc = no_pending(r);
+    if (prc) return prc;
+  }
   CK(hipSetDevice(r->device));
   (void)hipGetLastError(); // launch checks below must not report an earlier call's error
   int rc;
@@ -3207,9 +3295,15 @@ extern "C" int mbt_ring_potentials(const mbt_ring *r, int ib, double *out, doubl
   const size_t loff = (size_t)(r->Ph[ring].bucket_off + r->rh[ring].cfg.bucket[local]) * Nc;
   const double *src[7] = {r->sfL + off, r->lr_wake + loff, r->fnp_out + off, r->sfV + off, r->dipV_out + off,
                           r->sfH + off, r->dipH_out + off};
-  for (int k = 0; k < 7; k++) CK(hipMemcpy(out + k * Nc, src[k], Nc * sizeof(double), hipMemcpyDeviceToHost));
+  // eight small copies into pinned memory, one wait
+  double *stage = r->pot_host;
+  for (int k = 0; k < 7; k++)
+    CK(hipMemcpyAsync(stage + k * Nc, src[k], Nc * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  CK(hipMemcpyAsync(stage + 7 * Nc, r->ts + ring, sizeof(TurnState), hipMemcpyDeviceToHost, r->stream));
+  CK(hipStreamSynchronize(r->stream));
+  memcpy(out, stage, 7 * Nc * sizeof(double));
   TurnState ts;
-  CK(hipMemcpy(&ts, r->ts + ring, sizeof ts, hipMemcpyDeviceToHost));
+  memcpy(&ts, stage + 7 * Nc, sizeof ts);
   if (rfprev) {
     rfprev[0] = ts.phai0_prev;
     rfprev[1] = ts.Vrf0_prev;
@@ -3325,6 +3419,10 @@ extern "C" int mbt_ring_get_chunk(const mbt_ring *r) { return r->chunk_items; }
 // how the moment sums are grouped, i.e. results at the rounding level.
 extern "C" int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms)
 {
+  {
+    const int prc = no_pending(r);
+    if (prc) return prc;
+  }
   static const int cand[AUTOTUNE_CANDIDATES] = {1, 2, 4, 8, 16, 32, 64};
   CK(hipSetDevice(r->device));
   (void)hipGetLastError();

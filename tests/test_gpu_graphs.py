@@ -95,3 +95,37 @@ def test_graph_cache_keyed_by_monitor_offset(tmp_path):
     assert np.any(aa[14] != 0) and not np.any(aa[13] != 0)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "eager"])
+def test_async_blocks_equal_track(tmp_path, graphs):
+    """mbt_ring_track_async + mbt_ring_track_wait, blocks of uneven length with
+    potentials reads between them, give the bits mbt_ring_track gives; misuse is
+    refused with MBT_EINVAL and leaves the ring usable."""
+    path = write_variant("wakes", str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    a = m.Ring(cfg, graphs=graphs)
+    b = m.Ring(cfg, graphs=graphs)
+    for r in (a, b):
+        r.prepare()
+    amax = a.async_max()
+    assert 1 <= amax <= 256
+    for n in (9, 1, amax, 37):
+        a.track_async(n)
+        with pytest.raises(m.MbtError):
+            a.track_async(1)  # one block at a time
+        with pytest.raises(m.MbtError):
+            a.track(1)  # tracking while a block is pending
+        sa, aa, ra = a.track_wait()
+        sb, ab, rb = b.track(n)
+        assert np.array_equal(sa, sb) and np.array_equal(aa, ab) and np.array_equal(ra, rb), n
+        pa, pb = a.potentials(0), b.potentials(0)
+        assert all(np.array_equal(x, y) for x, y in zip(pa, pb))
+    with pytest.raises(m.MbtError):
+        a.track_wait()  # nothing pending
+    with pytest.raises(m.MbtError):
+        a.track_async(amax + 1)
+    assert np.array_equal(a.get_bunch(0), b.get_bunch(0))
+    assert np.array_equal(a.final_stats(), b.final_stats())
+    a.close()
+    b.close()
