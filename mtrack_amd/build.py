@@ -24,8 +24,12 @@ ARCH = "gfx950"
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-I" + INC, "-I" + CSRC]
+# -amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs (gfx950 allows both); the
+# default heuristic kept the RW GEMM's loop-carried accumulators in VGPRs and
+# copied them to and from AGPRs around every MFMA
 HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form=1",
             "-munsafe-fp-atomics", "-I" + INC, "-I" + CSRC, "-Wno-unused-result"]
 
 

@@ -1257,6 +1257,64 @@ struct RWGemm {
 // LDS bytes of one k_rwg_tile block: Y slice [4 per][16] + taps [4 per + HP - 1]
 static inline size_t rwg_lds(int per, int HP) { return sizeof(double) * ((size_t)68 * per + HP); }
 
+// The K loop of one wave over NG row tiles rt0, rt0 + 4, ... (all valid):
+// A[i][a] = c[a - i] read from the staged taps at al - i + HP - 1, B = the Y
+// slice.  Rows i >= h (up to HP) read taps too: their sums are never read.
+// F32: operands split into high and low floats, hi*hi + hi*lo + lo*hi in FP32,
+// summed in FP32 over RWG32_CHUNK steps and then in FP64.
+template <bool F32, int NG>
+__device__ __forceinline__ void rwg_rows(const double *Yf, const double *cs, int nA, int HP, int rt0, int kq,
+                                         int l16, double *part, int N, int j)
+{
+  mfma_d4 acc[NG];
+  mfma_f4 acc32[NG];
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+    acc[g] = (mfma_d4){0.0, 0.0, 0.0, 0.0};
+    acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  for (int st = 0; st < nA / 4; st++) {
+    const int al = 4 * st + kq;
+    const double bv = Yf[al * 16 + l16];
+    double avs[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) avs[g] = cs[al - ((rt0 + 4 * g) * 16 + l16) + HP - 1];
+    if (F32) {
+      const float bh = (float)bv, bl = (float)(bv - (double)bh);
+#pragma unroll
+      for (int g = 0; g < NG; g++) {
+        const float ah = (float)avs[g], alo = (float)(avs[g] - (double)ah);
+        acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(alo, bh, acc32[g], 0, 0, 0);
+        acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc32[g], 0, 0, 0);
+        acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc32[g], 0, 0, 0);
+      }
+      if ((st + 1) % RWG32_CHUNK == 0 || st == nA / 4 - 1) {
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[g][v] += (double)acc32[g][v];
+          acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < NG; g++) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(avs[g], bv, acc[g], 0, 0, 0);
+    }
+  }
+  // part: this K slice's partial sums [HP][N]; column j of this lane
+  if (j < N) {
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        // result layouts: f64 16x16x4 element v of lane l is row l / 16 + 4 v,
+        // f32 16x16x4 row 4 (l / 16) + v; column l % 16 in both
+        const int row = (rt0 + 4 * g) * 16 + (F32 ? 4 * kq + v : kq + 4 * v);
+        part[(size_t)row * N + j] = acc[g][v];
+      }
+  }
+}
+
 template <bool F32>
 __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__restrict__ Pr,
                                                 const TurnState *__restrict__ tsr, const double *rwhist,
@@ -1264,15 +1322,17 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
 {
   extern __shared__ double sm[];
   const int tj = blockIdx.x, ks = blockIdx.y, tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6, kq = lane >> 4, l16 = lane & 15;
+  // the wave index as a uniform value: the row-tile tests below are then
+  // scalar branches, not exec masks around the accumulators
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), kq = lane >> 4, l16 = lane & 15;
   const int h = Gm.h, M = Gm.M, HP = Gm.HP;
   const int ksteps = Gm.K / 4;
   const int s0 = ks * Gm.per, s1 = s0 + Gm.per < ksteps ? s0 + Gm.per : ksteps;
   const int A0 = 4 * s0, nA = 4 * (s1 - s0);
   double *Yf = sm;                        // [nA][16]
   double *cs = sm + (size_t)64 * Gm.per;  // taps m = A0 - HP + 1 + e
-  __shared__ long long ybase[16];         // history slot of row A0, per column
-  __shared__ int yring[16];
+  __shared__ long long ybase[16], yroff[16]; // history slot of row A0 and the ring's history offset, per column
+  __shared__ int yring[16], yboff[16];       // ring and its bucket offset, per column
   // ---- stage the Y slice (formed here from the history: Y[a][col] = dipole of
   // history slot (-rev h + a) mod n times the current of bucket a mod h, zero
   // for empty buckets and rows outside 1 <= a < h + M) and the taps
@@ -1283,6 +1343,8 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
     if (ring >= 0) {
       long long k0 = (-tsr[ring].rev * h + A0) % Gm.n;
       ybase[tid] = k0 < 0 ? k0 + Gm.n : k0;
+      yboff[tid] = Pr[ring].bucket_off;
+      yroff[tid] = Pr[ring].rw_off;
     }
   }
   __syncthreads();
@@ -1292,12 +1354,12 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
     const int a = A0 + t;
     double v = 0.0;
     if (ring >= 0 && a >= 1 && a < h + M) {
-      const DevParams &P = Pr[ring];
-      const int q = (q0 + t) % h;
-      if (nfFill[P.bucket_off + q]) {
+      int q = q0 + t; // bucket a mod h (q0 < h, t < 4 per)
+      while (q >= h) q -= h;
+      if (nfFill[yboff[jl] + q]) {
         long long idx = ybase[jl] + t;
         if (idx >= Gm.n) idx %= Gm.n; // t < 4 per may exceed n when h is small
-        v = rwhist[P.rw_off + 2 * idx + (jl & 1)] * Ibunch[P.bucket_off + q];
+        v = rwhist[yroff[jl] + 2 * idx + (jl & 1)] * Ibunch[yboff[jl] + q];
       }
     }
     Yf[e] = v;
@@ -1308,60 +1370,23 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
     cs[e] = (m >= 1 && m <= M) ? Gm.c[m] : 0.0;
   }
   __syncthreads();
-  // ---- row tiles out of LDS
+  // ---- row tiles out of LDS: wave wv takes tiles wv, wv + 4, ... in groups of
+  // up to RWG_RT; the group's tile count is a template argument of the K loop,
+  // so that the loop has no conditions and the accumulators stay in AGPRs
   const int nrt = HP / 16;
+  double *part = Gm.part + (size_t)ks * HP * Gm.N;
+  const int j = tj * 16 + l16;
   for (int rt0 = wv; rt0 < nrt; rt0 += 4 * RWG_RT) {
-    mfma_d4 acc[RWG_RT];
-    mfma_f4 acc32[RWG_RT];
-#pragma unroll
-    for (int g = 0; g < RWG_RT; g++) {
-      acc[g] = (mfma_d4){0.0, 0.0, 0.0, 0.0};
-      acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
-    }
-    for (int st = 0; st < nA / 4; st++) {
-      const int al = 4 * st + kq;
-      const double bv = Yf[al * 16 + l16];
-      const float bh = (float)bv, bl = (float)(bv - (double)bh);
-#pragma unroll
-      for (int g = 0; g < RWG_RT; g++) {
-        const int rt = rt0 + 4 * g;
-        if (rt < nrt) {
-          const int i = rt * 16 + l16;
-          // A[i][a] = c[a - i]: index a - i - mlo = al - i + HP - 1; rows i >= h are zero
-          const double av = i < h ? cs[al - i + HP - 1] : 0.0;
-          if (F32) {
-            const float ah = (float)av, alo = (float)(av - (double)ah);
-            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(alo, bh, acc32[g], 0, 0, 0);
-            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, acc32[g], 0, 0, 0);
-            acc32[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, acc32[g], 0, 0, 0);
-          } else {
-            acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[g], 0, 0, 0);
-          }
-        }
-      }
-      if (F32 && ((st + 1) % RWG32_CHUNK == 0 || st == nA / 4 - 1)) {
-#pragma unroll
-        for (int g = 0; g < RWG_RT; g++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc[g][v] += (double)acc32[g][v];
-          acc32[g] = (mfma_f4){0.0f, 0.0f, 0.0f, 0.0f};
-        }
-      }
-    }
-    const int j = tj * 16 + l16;
-    if (j < Gm.N) {
-#pragma unroll
-      for (int g = 0; g < RWG_RT; g++) {
-        const int rt = rt0 + 4 * g;
-        if (rt >= nrt) continue;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          // result layouts: f64 16x16x4 element v of lane l is row l / 16 + 4 v,
-          // f32 16x16x4 row 4 (l / 16) + v; column l % 16 in both
-          const int row = rt * 16 + (F32 ? 4 * kq + v : kq + 4 * v);
-          Gm.part[((size_t)ks * HP + row) * Gm.N + j] = acc[g][v];
-        }
-      }
+    const int ng = min(RWG_RT, (nrt - rt0 + 3) / 4);
+    switch (ng) {
+    case 1: rwg_rows<F32, 1>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 2: rwg_rows<F32, 2>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 3: rwg_rows<F32, 3>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 4: rwg_rows<F32, 4>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 5: rwg_rows<F32, 5>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 6: rwg_rows<F32, 6>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    case 7: rwg_rows<F32, 7>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
+    default: rwg_rows<F32, 8>(Yf, cs, nA, HP, rt0, kq, l16, part, Gm.N, j); break;
     }
   }
 }
