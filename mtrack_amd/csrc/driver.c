@@ -44,7 +44,6 @@ static FILE *wopen(const mbt_config *c, const char *name, const char *mode)
   return fp;
 }
 
-/* weak_writeout_bunch_distribution, src/bunch_weak.c:295-332 */
 /* turns per call of the tracking loop (the library's statistics block) */
 #define DRV_BLOCK 256
 
@@ -56,6 +55,7 @@ static int pot_due(const mbt_config *c, int nout, long t)
          t + 1 >= c->NrevPotentialsOut + (t / c->NrevScan) * c->NrevScan;
 }
 
+/* weak_writeout_bunch_distribution, src/bunch_weak.c:295-332 */
 static int write_distribution(const mbt_config *c, int plane, const double *soa6, int Np)
 {
   char name[3 * MBT_PATH];

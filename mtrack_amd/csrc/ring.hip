@@ -1372,7 +1372,8 @@ __global__ __launch_bounds__(256) void k_rwg_tile(RWGemm Gm, const DevParams *__
   __syncthreads();
   // ---- row tiles out of LDS: wave wv takes tiles wv, wv + 4, ... in groups of
   // up to RWG_RT; the group's tile count is a template argument of the K loop,
-  // so that the loop has no conditions and the accumulators stay in AGPRs
+  // so that the loop has no conditions and no accumulator moves between
+  // register files around its MFMAs
   const int nrt = HP / 16;
   double *part = Gm.part + (size_t)ks * HP * Gm.N;
   const int j = tj * 16 + l16;
