@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from variants import VARIANTS, conf_template, write_variant  # noqa: E402
+from variants import VARIANTS, conf_template, write_e2e_variant, write_variant  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "refharness")
 
@@ -91,8 +91,7 @@ def make_inp():
 
 
 # per-variant section overrides of the end-to-end cases (none at present)
-E2E_OVERRIDES = {}
-E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform", "rw_short", "wakefile"]
+E2E_CASES = ["wakes", "lr_fb", "rsis", "ws_uniform", "rw_short", "wakefile", "wakes_blocks", "wakes_mon300"]
 
 MPI_BIN = os.path.join(ROOT, "oracle", "_ref", "mbtrack-mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"
@@ -107,7 +106,7 @@ def make_e2e(name="wakes"):
     if name.startswith("ws_"):
         return make_e2e_ws(name)
     with tempfile.TemporaryDirectory() as d:
-        conf = write_variant(name, d, **E2E_OVERRIDES.get(name, {}))
+        conf = write_e2e_variant(name, d)
         work = os.path.join(d, "work")
         os.makedirs(work)
         st = np.loadtxt  # noqa: F841

@@ -12,7 +12,11 @@ set (src/input.c:465-478) and so divides by zero in every HOR/VER quantity
 them; for that case a reference nan must face a number here and every other
 token must match.  "rw_short" sets EnableRW_short (and so the reads): the
 short-range resistive wall in all three planes with the long-range kick, every
-token compared.
+token compared.  "wakefile" reads its Green functions from wake files.
+"wakes_blocks" and "wakes_mon300" run "wakes" for 600 turns with potentials
+only from turn 500 (so the program tracks long blocks first, then blocks split
+at every potentials output) and for 700 turns with a 300-turn monitor interval
+(longer than the library's 256-turn statistics block), with the same nan rule.
 
 "ws_uniform" is the weak-strong (BunchModel = STRONG) manager with FBII and the
 resistive wall, with the reference's bunch currents; its files were written by
@@ -38,7 +42,7 @@ import pytest
 
 import mtrack_amd as m
 from filecmp_util import compare
-from variants import write_variant, write_ws_variant
+from variants import write_e2e_variant, write_variant, write_ws_variant
 
 pytestmark = pytest.mark.gpu
 GOLD_ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -68,11 +72,12 @@ def compare_columns(a_text, b_text, rel):
     return ["column %d: %.3g > %g" % (c, e, rel) for c, e in enumerate(err) if e > rel]
 
 
-@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform", "rw_short", "wakefile"])
+@pytest.mark.parametrize("case", ["lr_fb", "rsis", "wakes", "ws_uniform", "rw_short", "wakefile", "wakes_blocks",
+                                  "wakes_mon300"])
 def test_cli_files_match_reference(tmp_path, case):
     gold = os.path.join(GOLD_ROOT, "e2e_" + case)
     d = str(tmp_path)
-    conf = write_ws_variant(case, d)[0] if case.startswith("ws_") else write_variant(case, d)
+    conf = write_ws_variant(case, d)[0] if case.startswith("ws_") else write_e2e_variant(case, d)
     work = os.path.join(d, "work")
     os.makedirs(work)
     print("case e2e", case, flush=True)
@@ -91,7 +96,7 @@ def test_cli_files_match_reference(tmp_path, case):
         else:
             if f.startswith("potentials_bunch_"):
                 g_text, m_text = mask_slice0_shape(g_text), mask_slice0_shape(m_text)
-            errs = compare(g_text, m_text, a_nan_matches_number=(case == "wakes"))
+            errs = compare(g_text, m_text, a_nan_matches_number=case.startswith("wakes"))
         if errs:
             bad[f] = errs[:3]
     assert not bad, bad

@@ -139,6 +139,24 @@ def write_wake_files(d):
     return out
 
 
+# end-to-end cases beyond the variants themselves: (variant, section overrides)
+E2E_VARIANTS = {
+    # 600 turns with potentials only from turn 500: the drop-in program tracks
+    # 240-turn blocks first, then blocks split at every potentials output
+    "wakes_blocks": ("wakes", {"tracking": {"NrevTot": 600, "NrevMon": 20, "NrevPotentialsOut": 500}}),
+    # a monitor interval longer than the library's 256-turn statistics block
+    "wakes_mon300": ("wakes", {"tracking": {"NrevTot": 700, "NrevMon": 300, "NrevPotentialsOut": 600}}),
+}
+
+
+def write_e2e_variant(name, d):
+    """A variant or an E2E_VARIANTS case written into d; returns the .conf path."""
+    if name in E2E_VARIANTS:
+        base, over = E2E_VARIANTS[name]
+        return write_variant(base, d, **over)
+    return write_variant(name, d)
+
+
 def write_variant(name, d, **section_overrides):
     """Writes the variant's .conf (and filling file) into directory d; returns the
     path.  section_overrides maps a section name to keys that replace the
