@@ -484,7 +484,7 @@ def main():
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production line (profiling of the other lines only)")
     ap.add_argument("--no-fbii", action="store_true", help="skip the fast beam-ion line")
-    ap.add_argument("--fbii-steps", type=int, default=10)
+    ap.add_argument("--fbii-steps", type=int, default=20)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
