@@ -349,6 +349,17 @@ void mbt_ws_destroy(mbt_ws *w);
  *   ion[t][ib][3]   ion cloud after bunch ib's passage: sum Nri x, sum Nri y, sum Nri
  *   zcm[t][kb]      slice-averaged z of every bucket at the end of the turn */
 int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, double *ion, double *zcm);
+/* mbt_ws_track split in two, so that a caller can write the previous turns'
+ * files while the next turns run: _async enqueues nturns turns
+ * (1 .. mbt_ws_async_max(w), one block of the device's output buffers) and
+ * returns; _wait waits for them and copies their outputs as mbt_ws_track does.
+ * One block at a time: mbt_ws_track and _async fail with MBT_EINVAL while one
+ * is pending, and so does _async with train timing on; mbt_ws_get_slices
+ * waits for the pending turns.  No counterpart in the reference, whose manager
+ * receives each turn's slices over MPI (src/manager_weakstrong.c:77-248). */
+long mbt_ws_async_max(const mbt_ws *w);
+int mbt_ws_track_async(mbt_ws *w, long nturns);
+int mbt_ws_track_wait(mbt_ws *w, double *cm, double *mon, double *ion, double *zcm);
 /* slice arrays of filled bunch ordinal ib: out[6][S] = tau, delta, x, x', z, z' */
 int mbt_ws_get_slices(const mbt_ws *w, int ib, double *out);
 /* macro-electrons per slice npop[S]; *nslices = S */

@@ -86,6 +86,10 @@ def lib():
         L.mbt_ws_create.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
         L.mbt_ws_destroy.argtypes = [vp]
         L.mbt_ws_track.argtypes = [vp, ctypes.c_long, dp, dp, dp, dp]
+        L.mbt_ws_async_max.argtypes = [vp]
+        L.mbt_ws_async_max.restype = ctypes.c_long
+        L.mbt_ws_track_async.argtypes = [vp, ctypes.c_long]
+        L.mbt_ws_track_wait.argtypes = [vp, dp, dp, dp, dp]
         L.mbt_ws_get_slices.argtypes = [vp, ip, dp]
         L.mbt_ws_population.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
         L.mbt_ws_distribution.argtypes = [vp, dp]
@@ -392,6 +396,25 @@ class WeakStrong:
         ion = np.zeros((nturns, self.Nb, 3))
         zcm = np.zeros((nturns, self.h))
         _check(lib().mbt_ws_track(self.ptr, nturns, dptr(cm), dptr(mon), dptr(ion), dptr(zcm)))
+        return cm, mon, ion, zcm
+
+    def async_max(self):
+        return lib().mbt_ws_async_max(self.ptr)
+
+    def track_async(self, nturns):
+        """Enqueues nturns turns (at most async_max()) without waiting."""
+        _check(lib().mbt_ws_track_async(self.ptr, nturns))
+        self._async_n = nturns
+
+    def track_wait(self):
+        """Waits for the turns of track_async; returns their outputs as track does."""
+        n = getattr(self, "_async_n", 0)
+        cm = np.zeros((n, self.Nb, 4))
+        mon = np.zeros((n, self.Nb, 18))
+        ion = np.zeros((n, self.Nb, 3))
+        zcm = np.zeros((n, self.h))
+        _check(lib().mbt_ws_track_wait(self.ptr, dptr(cm), dptr(mon), dptr(ion), dptr(zcm)))
+        self._async_n = 0
         return cm, mon, ion, zcm
 
     def set_timing(self, on=True):

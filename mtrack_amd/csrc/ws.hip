@@ -890,12 +890,26 @@ struct mbt_ws {
   hipEvent_t *ev;     // [2 WS_T] train start / end per turn of a block
   double train_ms;    // summed train time of the last mbt_ws_track call
   long train_launches;
+  long async_n;       // turns enqueued by mbt_ws_track_async, not yet collected (0: none)
 };
+
+// Tracking calls that would overwrite the output block of turns enqueued by
+// mbt_ws_track_async fail while those turns are pending.
+static int ws_no_pending(const mbt_ws *w)
+{
+  if (w->async_n) {
+    mbt_set_error("%ld turns enqueued by mbt_ws_track_async are pending: collect them with mbt_ws_track_wait",
+                  w->async_n);
+    return MBT_EINVAL;
+  }
+  return MBT_OK;
+}
 
 extern "C" void mbt_ws_destroy(mbt_ws *w)
 {
   if (!w) return;
   (void)hipSetDevice(w->device);
+  if (w->stream) (void)hipStreamSynchronize(w->stream); // turns of mbt_ws_track_async may still run
   void *p[] = {w->D.sl,  w->D.hist, w->D.Exy, w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
                w->D.done, w->D.cm,  w->D.mon,  w->D.ion, w->D.zcm,    w->d_bucket, w->d_fill,    w->d_npop,
                w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff};
@@ -1153,52 +1167,64 @@ static int ws_download(mbt_ws *w, long n, double *cm, double *mon, double *ion, 
   return MBT_OK;
 }
 
+// Enqueues n <= WS_T turns on the object's stream, their outputs in slots
+// 0 .. n-1 of the device's output block (with HIP events around each ion train
+// when timing is on).
+static int ws_enqueue(mbt_ws *w, long n)
+{
+  const int G = w->P.G, H = w->P.h;
+  const size_t lds2 = sizeof(double) * 2 * H;
+  CK(hipMemsetAsync(w->D.mon, 0, sizeof(double) * 18 * G * n, w->stream));
+  CK(hipMemsetAsync(w->D.ion, 0, sizeof(double) * 3 * G * n, w->stream));
+  for (long t = 0; t < n; t++) {
+    const long long rev = w->rev;
+    const int monitor = rev % w->cfg.NrevMon == 0;
+    hipLaunchKernelGGL(k_ws_phase1, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, rev, (int)t, monitor);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
+    CK(hipGetLastError());
+    if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
+    if (w->P.fbii && !w->serial) {
+      hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * w->P.S,
+                         w->stream, w->P, w->D);
+      CK(hipGetLastError());
+      if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
+      hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(KICK_WG), 0, w->stream, w->P, w->D, (int)t);
+      CK(hipGetLastError());
+    } else if (w->P.fbii) {
+      hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
+      CK(hipGetLastError());
+      for (int ib = 1; ib < G; ib++) {
+        const long long nF = (long long)ib * w->P.Nions;
+        const double Dtime =
+            fabs((double)(w->bucket_h[ib] - w->bucket_h[ib - 1])) * (w->cfg.T0 / w->cfg.h);
+        const int nblk = (int)((nF + WS_WG - 1) / WS_WG);
+        hipLaunchKernelGGL(k_ws_train, dim3(nblk), dim3(WS_WG), 0, w->stream, w->P, w->D, ib, nF, Dtime, (int)t);
+        CK(hipGetLastError());
+      }
+      if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
+    }
+    hipLaunchKernelGGL(k_ws_end, dim3((H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t);
+    CK(hipGetLastError());
+    w->rev++;
+  }
+  return MBT_OK;
+}
+
 extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, double *ion, double *zcm)
 {
+  int rc;
+  if ((rc = ws_no_pending(w))) return rc;
   CK(hipSetDevice(w->device));
   (void)hipGetLastError();
   const int G = w->P.G, H = w->P.h;
-  const size_t lds2 = sizeof(double) * 2 * H;
   long done = 0;
   w->train_ms = 0.0;
   w->train_launches = 0;
   while (done < nturns) {
     long n = nturns - done;
     if (n > WS_T) n = WS_T;
-    CK(hipMemsetAsync(w->D.mon, 0, sizeof(double) * 18 * G * n, w->stream));
-    CK(hipMemsetAsync(w->D.ion, 0, sizeof(double) * 3 * G * n, w->stream));
-    for (long t = 0; t < n; t++) {
-      const long long rev = w->rev;
-      const int monitor = rev % w->cfg.NrevMon == 0;
-      hipLaunchKernelGGL(k_ws_phase1, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, rev, (int)t, monitor);
-      CK(hipGetLastError());
-      hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
-      CK(hipGetLastError());
-      if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
-      if (w->P.fbii && !w->serial) {
-        hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * w->P.S,
-                           w->stream, w->P, w->D);
-        CK(hipGetLastError());
-        if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
-        hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(KICK_WG), 0, w->stream, w->P, w->D, (int)t);
-        CK(hipGetLastError());
-      } else if (w->P.fbii) {
-        hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
-        CK(hipGetLastError());
-        for (int ib = 1; ib < G; ib++) {
-          const long long nF = (long long)ib * w->P.Nions;
-          const double Dtime =
-              fabs((double)(w->bucket_h[ib] - w->bucket_h[ib - 1])) * (w->cfg.T0 / w->cfg.h);
-          const int nblk = (int)((nF + WS_WG - 1) / WS_WG);
-          hipLaunchKernelGGL(k_ws_train, dim3(nblk), dim3(WS_WG), 0, w->stream, w->P, w->D, ib, nF, Dtime, (int)t);
-          CK(hipGetLastError());
-        }
-        if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
-      }
-      hipLaunchKernelGGL(k_ws_end, dim3((H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t);
-      CK(hipGetLastError());
-      w->rev++;
-    }
+    if ((rc = ws_enqueue(w, n))) return rc;
     if (w->timing && w->P.fbii) {
       CK(hipStreamSynchronize(w->stream));
       for (long t = 0; t < n; t++) {
@@ -1208,12 +1234,49 @@ extern "C" int mbt_ws_track(mbt_ws *w, long nturns, double *cm, double *mon, dou
         w->train_launches++;
       }
     }
-    int rc = ws_download(w, n, cm ? cm + (size_t)done * G * 4 : NULL, mon ? mon + (size_t)done * G * 18 : NULL,
-                         ion ? ion + (size_t)done * G * 3 : NULL, zcm ? zcm + (size_t)done * H : NULL);
+    rc = ws_download(w, n, cm ? cm + (size_t)done * G * 4 : NULL, mon ? mon + (size_t)done * G * 18 : NULL,
+                     ion ? ion + (size_t)done * G * 3 : NULL, zcm ? zcm + (size_t)done * H : NULL);
     if (rc) return rc;
     done += n;
   }
   return MBT_OK;
+}
+
+extern "C" long mbt_ws_async_max(const mbt_ws *w)
+{
+  (void)w;
+  return WS_T;
+}
+
+extern "C" int mbt_ws_track_async(mbt_ws *w, long nturns)
+{
+  int rc;
+  if ((rc = ws_no_pending(w))) return rc;
+  if (nturns < 1 || nturns > WS_T) {
+    mbt_set_error("mbt_ws_track_async: 1 .. %d turns per call", WS_T);
+    return MBT_EINVAL;
+  }
+  if (w->timing) {
+    mbt_set_error("mbt_ws_track_async: not with train timing on");
+    return MBT_EINVAL;
+  }
+  CK(hipSetDevice(w->device));
+  (void)hipGetLastError(); // launch checks below must not report an earlier call's error
+  if ((rc = ws_enqueue(w, nturns))) return rc;
+  w->async_n = nturns;
+  return MBT_OK;
+}
+
+extern "C" int mbt_ws_track_wait(mbt_ws *w, double *cm, double *mon, double *ion, double *zcm)
+{
+  if (!w->async_n) {
+    mbt_set_error("mbt_ws_track_wait: no turns pending");
+    return MBT_EINVAL;
+  }
+  CK(hipSetDevice(w->device));
+  const long n = w->async_n;
+  w->async_n = 0;
+  return ws_download(w, n, cm, mon, ion, zcm);
 }
 
 extern "C" int mbt_ws_get_slices(const mbt_ws *w, int ib, double *out)

@@ -87,16 +87,21 @@ int mbt_manager_weakstrong(const mbt_config *c, int device, int flags)
   const size_t hn = (size_t)nmon * G + H + 2;
   double(*h_cm)[3] = calloc(hn, sizeof *h_cm), (*h_amp)[3] = calloc(hn, sizeof *h_amp);
   double(*h_ampcm)[3] = calloc(hn, sizeof *h_ampcm);
-  const long B = 256;
+  const long B = mbt_ws_async_max(w);
   double *mon = malloc(sizeof(double) * 18 * G * B), *ion = malloc(sizeof(double) * 3 * G * B);
   double *zcm = malloc(sizeof(double) * H * B);
   printf("Starting multi bunch tracking...\n");
   fflush(stdout);
   const char *pn[3] = {"LON", "HOR", "VER"};
-  long rev = 0;
-  while (rev < NrevTot) {
-    long n = NrevTot - rev < B ? NrevTot - rev : B;
-    if ((rc = mbt_ws_track(w, n, NULL, mon, ion, zcm))) break;
+  /* blocks of B turns: the next block is enqueued before the files of the
+     last one are written, so that the writing runs while the GPU tracks */
+  long rev = 0, pend = NrevTot < B ? NrevTot : B;
+  if (pend > 0) rc = mbt_ws_track_async(w, pend);
+  while (rc == MBT_OK && rev < NrevTot) {
+    const long n = pend;
+    if ((rc = mbt_ws_track_wait(w, NULL, mon, ion, zcm))) break;
+    pend = NrevTot - rev - n < B ? NrevTot - rev - n : B;
+    if (pend > 0 && (rc = mbt_ws_track_async(w, pend))) break;
     for (long t = 0; t < n; t++, rev++) {
       if (rev % 100 == 0) {
         if (rev + 99 >= NrevTot) printf("Transformation for turn %3ld to %3ld", rev, NrevTot - 1);

@@ -74,3 +74,36 @@ def test_ws_deterministic(tmp_path):
         g.close()
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def test_ws_async_blocks_equal_track(tmp_path):
+    """mbt_ws_track_async + mbt_ws_track_wait over blocks of uneven length give
+    the bits mbt_ws_track gives; misuse is refused with MBT_EINVAL and leaves the
+    object usable."""
+    path, mode = write_ws_variant("ws_fbii", str(tmp_path))
+    cfg = m.Config(path, str(tmp_path) + "/")
+    a = m.WeakStrong(cfg, bunch_currents=True)
+    b = m.WeakStrong(cfg, bunch_currents=True)
+    amax = a.async_max()
+    assert amax >= 1
+    for n in (5, 1, 12):
+        a.track_async(n)
+        with pytest.raises(m.MbtError):
+            a.track_async(1)  # one block at a time
+        with pytest.raises(m.MbtError):
+            a.track(1)  # tracking while a block is pending
+        outs_a, outs_b = a.track_wait(), b.track(n)
+        for x, y in zip(outs_a, outs_b):
+            assert np.array_equal(x, y), n
+    with pytest.raises(m.MbtError):
+        a.track_wait()  # nothing pending
+    with pytest.raises(m.MbtError):
+        a.track_async(amax + 1)
+    a.set_timing(True)
+    with pytest.raises(m.MbtError):
+        a.track_async(1)  # not with train timing on
+    a.set_timing(False)
+    for ib in range(a.Nb):
+        assert np.array_equal(a.slices(ib), b.slices(ib))
+    a.close()
+    b.close()
