@@ -83,6 +83,7 @@ struct WSP {
   double factBE, sxx, szz, deltT, aAx, aAz, aLx, aLz, den_dQ; // FBII fields
   double iaAx, iaAz;                                          // 1 / aAx, 1 / aAz
   double factF, factjp0, CdeltT;
+  double dtF; // deltT * factF: the ion train's velocity kick per unit field
 };
 
 struct WSD {
@@ -585,25 +586,33 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
 
 // The ion train's field table of a slice in LDS: the 5 x 5 grid with its edge
 // rows and columns repeated once more, 7 x 7 entries, entry 24 + 7 kx + kz for
-// kx, kz in -3 .. 3 = the point (clamp(kx), clamp(kz)) of the grid.  The
-// partner of the reference's cell k1 (one step further from the centre in the
-// direction of the coordinate's sign, clamped to the grid) is then the plain
-// next entry: no clamp of the partner, and rows of 7 spread neighbouring points
-// over the LDS banks.
+// kx, kz in -3 .. 3 = the point (clamp(kx), clamp(kz)) of the grid; rows of 7
+// spread neighbouring points over the LDS banks.
 #define WS_QT 49 // entries per slice
+#define WS_CH 24 // slices per staged chunk of a bunch's tables (a multiple of 4)
 
-// ws_interp on a slice's padded table: the same cells, fractions and steps.
-__device__ __forceinline__ void ws_interp_p(const double2 *t, double aAx, double aAz, double iax, double iaz,
-                                            double xx, double yy, double &Ex, double &Ey)
+// ws_interp on a slice's padded table, with the cell anchored at the floor of
+// the grid quotient u: the cell [f, f + 1], f = floor(u), at weight u - f.
+// The reference anchors at the grid point nearer zero (trunc(u)) and weighs
+// toward the next one out; for u >= 0 that is the same cell and steps, for
+// u < 0 the same bilinear interpolant taken from its other end.  The floor and
+// the weight come from t = u + 3 in one rounding (a fused multiply-add), as
+// its integer part and v_fract_f64: the weight then carries the rounding of t
+// (at most 4.4e-16 for |u| < 3) instead of that of u.  Quotients past the grid
+// take the cell [-3, -2] or [2, 3], whose outer point repeats the edge, so the
+// field there is the edge value, as the reference's clamped partner gives.
+// All four corners are fixed offsets (0, 1, 7, 8 entries) from one address:
+// no sign tests.
+__device__ __forceinline__ void ws_interp_p(const double2 *t, double iax, double iaz, double xx, double yy,
+                                            double &Ex, double &Ey)
 {
   // the bilinear steps as fused multiply-adds (one rounding less each)
 #pragma clang fp contract(fast)
-  const int k1x = min(max((int)(xx * iax), -2), 2);
-  const int k1z = min(max((int)(yy * iaz), -2), 2);
-  const int dx = xx >= 0 ? 7 : -7, dz = yy >= 0 ? 1 : -1;
-  const double2 *e = t + 24 + 7 * k1x + k1z;
-  const double2 e1 = e[0], e2 = e[dx], e3 = e[dz], e4 = e[dx + dz];
-  const double sx = fabs((xx - k1x * aAx) * iax), sz = fabs((yy - k1z * aAz) * iaz);
+  const double tx = fma(xx, iax, 3.0), tz = fma(yy, iaz, 3.0);
+  const double sx = __builtin_amdgcn_fract(tx), sz = __builtin_amdgcn_fract(tz);
+  const int kx = min(max((int)tx, 0), 5), kz = min(max((int)tz, 0), 5);
+  const double2 *e = t + 7 * kx + kz;
+  const double2 e1 = e[0], e2 = e[7], e3 = e[1], e4 = e[8];
   const double y5 = e1.y + sx * (e2.y - e1.y);
   const double y6 = e3.y + sx * (e4.y - e3.y);
   const double x5 = e1.x + sx * (e2.x - e1.x);
@@ -613,19 +622,19 @@ __device__ __forceinline__ void ws_interp_p(const double2 *t, double aAx, double
 }
 
 // One slice c of a bunch for the lane's ion: the field at the ion, the ion's
-// kick, and the ion's share Nri E of the slice's kick.
+// kick, and the ion's share Nri E of the slice's kick.  The velocity kick
+// deltT (factF E) of the reference is taken as (deltT factF) E, one fused
+// multiply-add with the product precomputed (a rounding-level change).
 __device__ __forceinline__ void ion_slice(const WSP &P, const double2 *qt, int c, double Nri, double &x, double &xd,
                                           double &y, double &yd, double &sx, double &sy)
 {
   // the ion's kick as fused multiply-adds
 #pragma clang fp contract(fast)
   double Ex, Ey;
-  ws_interp_p(qt + c * WS_QT, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
-  const double ax = P.factF * Ex;
-  xd = xd + P.deltT * ax;
+  ws_interp_p(qt + c * WS_QT, P.iaAx, P.iaAz, x, y, Ex, Ey);
+  xd = xd + P.dtF * Ex;
   x = x + P.deltT * xd;
-  const double ay = P.factF * Ey;
-  yd = yd + P.deltT * ay;
+  yd = yd + P.dtF * Ey;
   y = y + P.deltT * yd;
   sx = Nri * Ex;
   sy = Nri * Ey;
@@ -633,12 +642,12 @@ __device__ __forceinline__ void ion_slice(const WSP &P, const double2 *qt, int c
 
 // One group of four slices c0 .. c0 + 3 of a bunch for the lane's ion: the
 // kicks, then the group's eight field sums (Nri Ex, Nri Ey per slice) reduced
-// over the wave into wred.  ALL: every lane's ion is in the cloud (lanes past
-// the last ion carry Nri = 0 from a finite state, so they add exact zeros);
-// FULL: all four slices exist.
+// over the wave into wred.  qt holds the tables of the slices from cq on.
+// ALL: every lane's ion is in the cloud (lanes past the last ion carry Nri = 0
+// from a finite state, so they add exact zeros); FULL: all four slices exist.
 template <bool ALL, bool FULL>
-__device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int c0, int S, bool act, double Nri,
-                                           double &x, double &xd, double &y, double &yd,
+__device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int cq, int c0, int S, bool act,
+                                           double Nri, double &x, double &xd, double &y, double &yd,
                                            double (*wred)[WS_WG / 64], int lane, int wv)
 {
   double v[8];
@@ -646,7 +655,7 @@ __device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int 
   for (int k = 0; k < 4; k++) {
     const int c = c0 + k;
     double sx = 0.0, sy = 0.0;
-    if ((ALL || act) && (FULL || c < S)) ion_slice(P, qt, c, Nri, x, xd, y, yd, sx, sy);
+    if ((ALL || act) && (FULL || c < S)) ion_slice(P, qt, c - cq, Nri, x, xd, y, yd, sx, sy);
     v[2 * k] = sx;
     v[2 * k + 1] = sy;
   }
@@ -658,7 +667,7 @@ __device__ __forceinline__ void ions_group(const WSP &P, const double2 *qt, int 
 // The last r = 1 or 2 slices c0 .. c0 + r - 1 of a bunch and, after them, the
 // cloud sums Nri x, Nri y, Nri: one eight-value reduction for both.
 template <bool ALL>
-__device__ __forceinline__ void ions_tail(const WSP &P, const double2 *qt, int c0, int r, int S, bool act,
+__device__ __forceinline__ void ions_tail(const WSP &P, const double2 *qt, int cq, int c0, int r, int S, bool act,
                                           double Nri, double &x, double &xd, double &y, double &yd,
                                           double (*wred)[WS_WG / 64], int lane, int wv)
 {
@@ -666,7 +675,7 @@ __device__ __forceinline__ void ions_tail(const WSP &P, const double2 *qt, int c
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     double sx = 0.0, sy = 0.0;
-    if ((ALL || act) && k < r) ion_slice(P, qt, c0 + k, Nri, x, xd, y, yd, sx, sy);
+    if ((ALL || act) && k < r) ion_slice(P, qt, c0 + k - cq, Nri, x, xd, y, yd, sx, sy);
     v[2 * k] = sx;
     v[2 * k + 1] = sy;
   }
@@ -684,27 +693,24 @@ __device__ __forceinline__ void ions_tail(const WSP &P, const double2 *qt, int c
   }
 }
 
-// One bunch ib of the train for the lane's ion (drift, then every slice),
-// followed by the cloud sums Nri x, Nri y, Nri over the wave into wred.
+// Slices cb .. ce - 1 of a bunch for the lane's ion (qt: their tables from cb
+// on; cb a multiple of 4) and, when they end the bunch (ce = S), the cloud
+// sums Nri x, Nri y, Nri after it, over the wave into wred.
 template <bool ALL>
-__device__ __forceinline__ void ions_bunch(const WSP &P, const WSD &D, const double2 *qt, int ib, bool act,
-                                           double Nri, double &x, double &xd, double &y, double &yd,
+__device__ __forceinline__ void ions_chunk(const WSP &P, const double2 *qt, int cb, int ce, bool act, double Nri,
+                                           double &x, double &xd, double &y, double &yd,
                                            double (*wred)[WS_WG / 64], int lane, int wv)
 {
   const int S = P.S;
-  if (ALL || act) {
-    const double dt = D.dtime[ib];
-    x = x + dt * xd;
-    y = y + dt * yd;
-  }
-  int c0 = 0;
-  for (; c0 + 4 <= S; c0 += 4) ions_group<ALL, true>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  int c0 = cb;
+  for (; c0 + 4 <= ce; c0 += 4) ions_group<ALL, true>(P, qt, cb, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  if (ce < S) return;
   const int r = S - c0;
   if (r == 1 || r == 2) {
-    ions_tail<ALL>(P, qt, c0, r, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+    ions_tail<ALL>(P, qt, cb, c0, r, S, act, Nri, x, xd, y, yd, wred, lane, wv);
     return;
   }
-  if (r) ions_group<ALL, false>(P, qt, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
+  if (r) ions_group<ALL, false>(P, qt, cb, c0, S, act, Nri, x, xd, y, yd, wred, lane, wv);
   double s0 = ALL || act ? Nri * x : 0.0, s1 = ALL || act ? Nri * y : 0.0, s2 = ALL || act ? Nri : 0.0;
   s0 = wsum64(s0);
   s1 = wsum64(s1);
@@ -722,8 +728,11 @@ __device__ __forceinline__ void ions_bunch(const WSP &P, const WSD &D, const dou
 // slice sums of Nri * E and the cloud sums Nri x, Nri y, Nri after the bunch,
 // each the fixed-order sum of its four waves' sums.  The bunches up to the
 // block's last creator (some lanes' ions not yet made) run with per-lane
-// predicates, the rest without.  Dynamic LDS: S * WS_QT double2 (the padded
-// tables of one bunch).
+// predicates, the rest without.  The padded tables of a bunch are staged in
+// chunks of WS_CH slices (dynamic LDS: min(S, WS_CH) * WS_QT double2, 18.8 KB
+// at most), so that six blocks fit a CU: 78 registers a lane allow six waves
+// per SIMD, and the 1,422 blocks of the 356-bunch train are all resident at
+// once.
 __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
 {
   extern __shared__ double2 qt_s[];
@@ -732,16 +741,17 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
   const size_t GP = (size_t)P.G * npts;
   const long long nI = (long long)(P.G - 1) * npts;
   // Dispatch order: a block's work is the number of bunches after its ions'
-  // creator, longest first.  The first 1,024 blocks (4 per CU on 256 CUs) are
-  // dealt to the CUs round-robin, so blocks c, c + 256, c + 512 and c + 768
-  // share a CU: they take the logical blocks c, 511 - c, 512 + c and
-  // 1023 - c, the same total work on every CU instead of the four longest
-  // trains on the first (measured 4 % on the 356-bunch train).  Every logical
-  // block computes and writes what it would at its own index.
+  // creator, longest first.  Blocks are dealt to the CUs round-robin, so
+  // blocks c, c + 256, c + 512, ... share a CU: in each run of 512 blocks
+  // (or the last, shorter run, if it holds more than 256) block c takes the
+  // logical block 255 - c and block c + 256 the logical block 256 + c, so
+  // that every CU holds pairs of the same total work instead of the longest
+  // trains on the first CUs (measured 4 % on the 356-bunch train).  Every
+  // logical block computes and writes what it would at its own index.
   int L = blockIdx.x;
-  if (gridDim.x >= 1024 && L < 1024) {
-    const int c = L & 255, k = L >> 8;
-    L = k == 0 ? c : k == 1 ? 511 - c : k == 2 ? 512 + c : 1023 - c;
+  {
+    const int c = L & 255, k = L >> 8, base = (k >> 1) * 512;
+    if ((int)gridDim.x - base > 256) L = (k & 1) ? base + 256 + c : base + 255 - c;
   }
   const long long q0 = (long long)L * WS_WG, q = q0 + tid;
   const int jmin = (int)(q0 / npts);
@@ -757,19 +767,28 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
     yd = D.li[4 * GP + q];
   }
   // threads 0 .. 4 WS_QT - 1 stage entry r of the tables of slices sl0,
-  // sl0 + 4, ...: a fixed grid point per thread
+  // sl0 + 4, ... of a chunk: a fixed grid point per thread
   const int sl0 = tid / WS_QT, r = tid % WS_QT;
   const int src = 12 + 5 * min(max(r / 7 - 3, -2), 2) + min(max(r % 7 - 3, -2), 2);
   for (int ib = jmin + 1; ib < P.G; ib++) {
-    __syncthreads(); // the previous bunch's table and sums have been read
+    const bool act = ib > jmax || j < ib;
+    if (act) {
+      const double dt = D.dtime[ib];
+      x = x + dt * xd;
+      y = y + dt * yd;
+    }
     const double2 *exy = D.Exy + (size_t)ib * npts + src;
-    if (tid < 4 * WS_QT)
-      for (int c = sl0; c < S; c += 4) qt_s[c * WS_QT + r] = exy[c * 25];
-    __syncthreads();
-    if (ib <= jmax)
-      ions_bunch<false>(P, D, qt_s, ib, j < ib, Nri, x, xd, y, yd, wred, lane, wv);
-    else
-      ions_bunch<true>(P, D, qt_s, ib, true, Nri, x, xd, y, yd, wred, lane, wv);
+    for (int cb = 0; cb < S; cb += WS_CH) {
+      const int ce = min(S, cb + WS_CH);
+      __syncthreads(); // the previous chunk's tables (and bunch's sums) have been read
+      if (tid < 4 * WS_QT)
+        for (int c = cb + sl0; c < ce; c += 4) qt_s[(c - cb) * WS_QT + r] = exy[c * 25];
+      __syncthreads();
+      if (ib <= jmax)
+        ions_chunk<false>(P, qt_s, cb, ce, act, Nri, x, xd, y, yd, wred, lane, wv);
+      else
+        ions_chunk<true>(P, qt_s, cb, ce, true, Nri, x, xd, y, yd, wred, lane, wv);
+    }
     __syncthreads();
     // quantity-major, so that k_ws_kick reads each quantity's partials contiguously
     const long long nbi = min(((long long)ib * npts + WS_WG - 1) / WS_WG, (long long)D.nblk_ions);
@@ -1062,6 +1081,7 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
     P.factF = K_E / (28.0 * 1.836e3 * 9.1e-31);
     P.factjp0 = 1e+2 * 4.0 * M_PI * K_EPS0 * 2.819e-15 / c->Gamma;
     P.CdeltT = K_C * P.deltT;
+    P.dtF = P.deltT * P.factF;
     w->nelec_h = (double *)malloc(sizeof(double) * G);
     double *conv = (double *)malloc(sizeof(double) * G);
     for (int b = 0; b < G; b++) {
@@ -1185,7 +1205,7 @@ static int ws_enqueue(mbt_ws *w, long n)
     CK(hipGetLastError());
     if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
     if (w->P.fbii && !w->serial) {
-      hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * w->P.S,
+      hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * min(w->P.S, WS_CH),
                          w->stream, w->P, w->D);
       CK(hipGetLastError());
       if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));

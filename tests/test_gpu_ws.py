@@ -16,8 +16,9 @@ rounding-level change alone in the oracle itself
 (tests/test_oracle_ws_sensitivity.py: reciprocal grid quotients, tree sums,
 one-ulp libm results), so the bound there is RTOL_H416 = 2e-11, the largest of
 those (the GPU, whose ion train also fuses the interpolation's and the kicks'
-multiply-adds, measures 9.4e-12 ion-parallel and 1.1e-11 relay).  Runs are
-bitwise reproducible (fixed-order reductions).
+multiply-adds and anchors its cells at the floor of the grid quotient,
+measures 1.1e-11 in both train forms).  Runs are bitwise reproducible
+(fixed-order reductions).
 """
 import numpy as np
 import pytest
