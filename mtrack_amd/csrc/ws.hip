@@ -8,11 +8,12 @@
 //   k_ws_phase1  one block per bunch: synchrotron step of the centroids
 //                (transformSbunch_shortrange, src/rw_weakstrong.c:10-67), bunch
 //                centroid, CM history append (src/bunch.c:94-115), monitor-turn
-//                statistics (src/bunch.c:516-550); with FBII the Bassetti-Erskine
-//                field of every slice on the 5 x 5 absolute grid (two Faddeeva
-//                evaluations per point, wavefront-parallel) and the ions the
-//                bunch creates, kicked by its later slices
-//                (fbi_weakstrong_bunch, src/fbi_weakstrong.c:49-298);
+//                statistics (src/bunch.c:516-550); with FBII, one thread per
+//                point, the Bassetti-Erskine field of every slice on the 5 x 5
+//                absolute grid (two Faddeeva evaluations per point), then
+//                k_ws_create: the ions each bunch creates, one thread each,
+//                kicked by the bunch's later slices (fbi_weakstrong_bunch,
+//                src/fbi_weakstrong.c:49-298);
 //   k_ws_phase2  one block per bunch: betatron rotation of the centroids
 //                (transformSbunch_optic, :194-253) and the resistive-wall kick of
 //                every other bucket's centroid history (transformSbunch_longrange,
@@ -26,11 +27,13 @@
 //                and on the drift times, never on the kicks the train gives the
 //                electrons, so the whole relay is one launch with one lane per
 //                ion over every later bunch (k_ws_ions: per-slice wave sums,
-//                block partials per bunch) and one per-bunch reduction that
-//                applies the kicks (k_ws_kick); k_ws_cloud_reset + k_ws_train
-//                keep the bunch-by-bunch relay (one launch per bunch) for
-//                MBT_WS_SERIAL_TRAIN and for trains whose partials would not fit;
-//   k_ws_end     slice-averaged vertical centroid of every bucket.
+//                block partials per bunch), a per-bunch, per-quantity sum of
+//                the partials (k_ws_ksum) and the kicks (in k_ws_end);
+//                k_ws_cloud_reset + k_ws_train keep the bunch-by-bunch relay
+//                (one launch per bunch) for MBT_WS_SERIAL_TRAIN and for trains
+//                whose partials would not fit;
+//   k_ws_end     the ion-parallel train's kicks and cloud sums, and the
+//                slice-averaged vertical centroid of every bucket.
 //
 // Everything but the ion-field sums, sin and the Faddeeva function keeps the
 // reference's operation order; constants derived with libm are computed on the
@@ -47,11 +50,11 @@
 
 #define WS_S 50       // slice capacity of the reference's bunch_strong_t
 #define WS_PTS 1250   // 50 slices x 25 grid points
+#define WS_TQ (2 * WS_S + 3 + 12) // per bunch in WSD::tot: 2 S + 3 train sums, 4 x 3 own-ion sums
 #define WS_HIST 30    // CM history depth (src/bunch.h:155)
 #define WS_MSG 5000   // doubles of the history message (src/manager_weakstrong.c:170-173)
 #define WS_T 256      // turns of outputs per device buffer block
 #define WS_WG 256
-#define KICK_WG 1024 // k_ws_kick
 // device memory the ion-parallel train may use for its per-bunch partials
 #define WS_TRAIN_PARTIAL_MAX (8LL << 30)
 
@@ -104,6 +107,7 @@ struct WSD {
   double *tpart;          // ion-parallel train: per bunch ib, [2 S + 3][blocks with an ion ahead of ib]
   const long long *toff;  // [G] offset of bunch ib's partials in tpart
   int nblk_ions;          // blocks of k_ws_ions
+  double *tot;            // ion-parallel train: [G][WS_TQ] per-bunch sums of the partials and own-ion sums
   double *cm, *mon, *ion, *zcm; // output blocks [WS_T][G][4|18|3], [WS_T][h]
 };
 
@@ -237,129 +241,143 @@ __device__ __forceinline__ void ws_interp(const double2 *t, double aAx, double a
 
 // ------------------------------------------------------------------- phase 1
 
+// Per bunch (blocks 0 .. G-1): the synchrotron step of every slice, the
+// centroids it reports, the manager's history and the monitor statistics.
+// With FBII every thread of the grid (G S 25 / WS_WG blocks, at least G) also
+// evaluates the Bassetti-Erskine field at one absolute grid point of one slice
+// of one bunch; the field needs the slices' transverse positions only, which
+// the synchrotron step leaves alone.
 __global__ __launch_bounds__(WS_WG) void k_ws_phase1(WSP P, WSD D, long long rev, int slot, int monitor)
 {
-  __shared__ double2 exy_s[WS_PTS];
-  const int b = blockIdx.x, tid = threadIdx.x, S = P.S;
-  const int kb = D.bucket[b];
-  double *xtau = D.sl + (size_t)b * 6 * S, *xeps = xtau + S, *xx = xeps + S, *xp = xx + S, *zz = xp + S,
-         *zp = zz + S;
-  for (int ic = tid; ic < S; ic += WS_WG) {
-    const double e0 = xeps[ic];
-    const double gain = P.eps * sin(P.wrf * xtau[ic] + P.phai0);
-    xeps[ic] = e0 + gain - P.Urad;
-    xtau[ic] = xtau[ic] - e0 * P.T0 * P.ac;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double pos[3], slope[3];
-    ws_cm(xtau, xeps, D.npop, S, pos[0], slope[0]);
-    ws_cm(xx, xp, D.npop, S, pos[1], slope[1]);
-    ws_cm(zz, zp, D.npop, S, pos[2], slope[2]);
-    double *o = D.cm + ((size_t)slot * P.G + b) * 4;
-    o[0] = pos[1];
-    o[1] = slope[1];
-    o[2] = pos[2];
-    o[3] = slope[2];
-    // the manager's history of this bucket: shift by one, newest at 0
-    const double nw[4] = {pos[1], slope[1], pos[2], slope[2]};
-    for (int a = 0; a < 4; a++) {
-      double *hrow = D.hist + ((size_t)a * P.h + kb) * WS_HIST;
-      for (int i = 0; i < P.NmultiT; i++) {
-        const int j = P.NmultiT - i - 2;
-        if (j >= 0) hrow[j + 1] = hrow[j];
-        else hrow[0] = nw[a];
-      }
+  const int tid = threadIdx.x, S = P.S;
+  if (blockIdx.x < P.G) {
+    const int b = blockIdx.x;
+    const int kb = D.bucket[b];
+    double *xtau = D.sl + (size_t)b * 6 * S, *xeps = xtau + S, *xx = xeps + S, *xp = xx + S, *zz = xp + S,
+           *zp = zz + S;
+    for (int ic = tid; ic < S; ic += WS_WG) {
+      const double e0 = xeps[ic];
+      const double gain = P.eps * sin(P.wrf * xtau[ic] + P.phai0);
+      xeps[ic] = e0 + gain - P.Urad;
+      xtau[ic] = xtau[ic] - e0 * P.T0 * P.ac;
     }
-    if (monitor) {
-      double *m = D.mon + ((size_t)slot * P.G + b) * 18;
-      const double *U[3] = {xtau, xx, zz}, *Q[3] = {xeps, xp, zp};
-      for (int pl = 0; pl < 3; pl++) {
-        for (int k = 0; k < 6; k++) m[3 * k + pl] = 0.0;
-        if (!P.track[pl]) continue;
-        const double *u = U[pl], *p = Q[pl];
-        double avr, avs;
-        ws_cm(u, p, D.npop, S, avr, avs);
-        double sq = 0.0, sqs = 0.0, amp = 0.0;
-        int w1 = 0, w2 = 0, n = 0;
-        for (int ic = 0; ic < S; ic++) {
-          sq += (u[ic] - avr) * (u[ic] - avr) * D.npop[ic];
-          w1 += D.npop[ic];
+    __syncthreads();
+    if (tid == 0) {
+      double pos[3], slope[3];
+      ws_cm(xtau, xeps, D.npop, S, pos[0], slope[0]);
+      ws_cm(xx, xp, D.npop, S, pos[1], slope[1]);
+      ws_cm(zz, zp, D.npop, S, pos[2], slope[2]);
+      double *o = D.cm + ((size_t)slot * P.G + b) * 4;
+      o[0] = pos[1];
+      o[1] = slope[1];
+      o[2] = pos[2];
+      o[3] = slope[2];
+      // the manager's history of this bucket: shift by one, newest at 0
+      const double nw[4] = {pos[1], slope[1], pos[2], slope[2]};
+      for (int a = 0; a < 4; a++) {
+        double *hrow = D.hist + ((size_t)a * P.h + kb) * WS_HIST;
+        for (int i = 0; i < P.NmultiT; i++) {
+          const int j = P.NmultiT - i - 2;
+          if (j >= 0) hrow[j + 1] = hrow[j];
+          else hrow[0] = nw[a];
         }
-        for (int ic = 0; ic < S; ic++) {
-          sqs += (p[ic] - avs) * (p[ic] - avs) * D.npop[ic];
-          w2 += D.npop[ic];
+      }
+      if (monitor) {
+        double *m = D.mon + ((size_t)slot * P.G + b) * 18;
+        const double *U[3] = {xtau, xx, zz}, *Q[3] = {xeps, xp, zp};
+        for (int pl = 0; pl < 3; pl++) {
+          for (int k = 0; k < 6; k++) m[3 * k + pl] = 0.0;
+          if (!P.track[pl]) continue;
+          const double *u = U[pl], *p = Q[pl];
+          double avr, avs;
+          ws_cm(u, p, D.npop, S, avr, avs);
+          double sq = 0.0, sqs = 0.0, amp = 0.0;
+          int w1 = 0, w2 = 0, n = 0;
+          for (int ic = 0; ic < S; ic++) {
+            sq += (u[ic] - avr) * (u[ic] - avr) * D.npop[ic];
+            w1 += D.npop[ic];
+          }
+          for (int ic = 0; ic < S; ic++) {
+            sqs += (p[ic] - avs) * (p[ic] - avs) * D.npop[ic];
+            w2 += D.npop[ic];
+          }
+          for (int ic = 0; ic < S; ic++) {
+            amp += D.npop[ic] * (P.gamma1[pl] * (u[ic] * u[ic]) + 2.0 * P.alpha1[pl] * u[ic] * p[ic] +
+                                 P.beta1[pl] * (p[ic] * p[ic]));
+            n += D.npop[ic];
+          }
+          m[pl] = avr;
+          m[3 + pl] = sqrt(sq / ((double)w1));
+          m[6 + pl] = avs;
+          m[9 + pl] = sqrt(sqs / ((double)w2));
+          m[12 + pl] = amp / ((double)n);
+          m[15 + pl] = P.gamma1[pl] * (avr * avr) + 2.0 * P.alpha1[pl] * avr * avs + P.beta1[pl] * (avs * avs);
         }
-        for (int ic = 0; ic < S; ic++) {
-          amp += D.npop[ic] * (P.gamma1[pl] * (u[ic] * u[ic]) + 2.0 * P.alpha1[pl] * u[ic] * p[ic] +
-                               P.beta1[pl] * (p[ic] * p[ic]));
-          n += D.npop[ic];
-        }
-        m[pl] = avr;
-        m[3 + pl] = sqrt(sq / ((double)w1));
-        m[6 + pl] = avs;
-        m[9 + pl] = sqrt(sqs / ((double)w2));
-        m[12 + pl] = amp / ((double)n);
-        m[15 + pl] = P.gamma1[pl] * (avr * avr) + 2.0 * P.alpha1[pl] * avr * avs + P.beta1[pl] * (avs * avs);
       }
     }
   }
   if (!P.fbii) return;
-  // ---- Bassetti-Erskine field of every slice at the absolute grid points
-  const double nel = D.nelec[b];
+  // ---- Bassetti-Erskine field of one slice at one absolute grid point
   const int npts = S * 25;
-  for (int q = tid; q < npts; q += WS_WG) {
-    const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
-    const double dQ = nel * D.npop[cell] * K_E;
-    const double fdQ = dQ / P.den_dQ;
-    const double Dx = msx * P.aAx - xx[cell];
-    const double xi1 = Dx / P.factBE, xi2 = xi1 * P.szz / P.sxx;
-    const double Dy = msz * P.aAz - zz[cell];
-    const double yi1 = Dy / P.factBE, yi2 = yi1 * P.sxx / P.szz;
-    double u1, v1, u2, v2;
-    if (P.mixed) {
-      float a1, b1, a2, b2;
-      mbt_wofz_dev<float>((float)xi1, (float)yi1, a1, b1);
-      mbt_wofz_dev<float>((float)xi2, (float)yi2, a2, b2);
-      u1 = a1;
-      v1 = b1;
-      u2 = a2;
-      v2 = b2;
-    } else {
-      mbt_wofz_dev<double>(xi1, yi1, u1, v1);
-      mbt_wofz_dev<double>(xi2, yi2, u2, v2);
-    }
-    const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
-    const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
-    exy_s[q] = make_double2(Ex, Ey);
-    D.Exy[(size_t)b * npts + q] = make_double2(Ex, Ey);
+  const long long gq = (long long)blockIdx.x * WS_WG + tid;
+  if (gq >= (long long)P.G * npts) return;
+  const int b = (int)(gq / npts), q = (int)(gq - (long long)b * npts);
+  const double *xx = D.sl + (size_t)b * 6 * S + 2 * S, *zz = xx + 2 * S;
+  const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
+  const double dQ = D.nelec[b] * D.npop[cell] * K_E;
+  const double fdQ = dQ / P.den_dQ;
+  const double Dx = msx * P.aAx - xx[cell];
+  const double xi1 = Dx / P.factBE, xi2 = xi1 * P.szz / P.sxx;
+  const double Dy = msz * P.aAz - zz[cell];
+  const double yi1 = Dy / P.factBE, yi2 = yi1 * P.sxx / P.szz;
+  double u1, v1, u2, v2;
+  if (P.mixed) {
+    float a1, b1, a2, b2;
+    mbt_wofz_dev<float>((float)xi1, (float)yi1, a1, b1);
+    mbt_wofz_dev<float>((float)xi2, (float)yi2, a2, b2);
+    u1 = a1;
+    v1 = b1;
+    u2 = a2;
+    v2 = b2;
+  } else {
+    mbt_wofz_dev<double>(xi1, yi1, u1, v1);
+    mbt_wofz_dev<double>(xi2, yi2, u2, v2);
   }
-  __syncthreads();
-  // ---- the ions this bunch creates: one per local grid point, kicked by the
-  // bunch's later slices
-  const double conv = D.convion[b];
+  const double g = exp(-(Dx * Dx * 1.0e+06 / (2 * P.sxx * P.sxx) + Dy * Dy * 1.0e+06 / (2 * P.szz * P.szz)));
+  const double Ex = -fdQ * (v1 - g * v2), Ey = -fdQ * (u1 - g * u2);
+  D.Exy[gq] = make_double2(Ex, Ey);
+}
+
+// The ions every bunch creates, one per thread: one per local grid point of
+// each slice, kicked by the bunch's later slices (their field tables read from
+// the one k_ws_phase1 wrote).
+__global__ __launch_bounds__(WS_WG) void k_ws_create(WSP P, WSD D)
+{
+  const int S = P.S, npts = S * 25;
+  const long long gq = (long long)blockIdx.x * WS_WG + threadIdx.x;
+  if (gq >= (long long)P.G * npts) return;
+  const int b = (int)(gq / npts), q = (int)(gq - (long long)b * npts);
+  const double *xx = D.sl + (size_t)b * 6 * S + 2 * S, *zz = xx + 2 * S;
+  const double2 *exy = D.Exy + (size_t)b * npts;
+  const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
+  const double Nri = D.convion[b] * ((double)D.nofe[q]);
+  double x = xx[cell] + msx * P.aLx, xd = 0.0, y = zz[cell] + msz * P.aLz, yd = 0.0;
+  for (int c = cell + 1; c < S; c++) {
+    double Ex, Ey;
+    ws_interp(exy + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
+    const double ax = P.factF * Ex;
+    xd = xd + P.deltT * ax;
+    x = x + P.deltT * xd;
+    const double ay = P.factF * Ey;
+    yd = yd + P.deltT * ay;
+    y = y + P.deltT * yd;
+  }
   const size_t GP = (size_t)P.G * npts;
-  for (int q = tid; q < npts; q += WS_WG) {
-    const int cell = q / 25, r = q % 25, msx = r / 5 - 2, msz = r % 5 - 2;
-    const double Nri = conv * ((double)D.nofe[q]);
-    double x = xx[cell] + msx * P.aLx, xd = 0.0, y = zz[cell] + msz * P.aLz, yd = 0.0;
-    for (int c = cell + 1; c < S; c++) {
-      double Ex, Ey;
-      ws_interp(exy_s + c * 25, P.aAx, P.aAz, P.iaAx, P.iaAz, x, y, Ex, Ey);
-      const double ax = P.factF * Ex;
-      xd = xd + P.deltT * ax;
-      x = x + P.deltT * xd;
-      const double ay = P.factF * Ey;
-      yd = yd + P.deltT * ay;
-      y = y + P.deltT * yd;
-    }
-    const size_t o = (size_t)b * npts + q;
-    D.li[o] = Nri;
-    D.li[GP + o] = x;
-    D.li[2 * GP + o] = xd;
-    D.li[3 * GP + o] = y;
-    D.li[4 * GP + o] = yd;
-  }
+  D.li[gq] = Nri;
+  D.li[GP + gq] = x;
+  D.li[2 * GP + gq] = xd;
+  D.li[3 * GP + gq] = y;
+  D.li[4 * GP + gq] = yd;
 }
 
 // ------------------------------------------------------------------- phase 2
@@ -584,25 +602,27 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
   }
 }
 
-// The ion train's field table of a slice in LDS: the 5 x 5 grid with its edge
-// rows and columns repeated once more, 7 x 7 entries, entry 24 + 7 kx + kz for
-// kx, kz in -3 .. 3 = the point (clamp(kx), clamp(kz)) of the grid; rows of 7
-// spread neighbouring points over the LDS banks.
-#define WS_QT 49 // entries per slice
-#define WS_CH 24 // slices per staged chunk of a bunch's tables (a multiple of 4)
+// The ion train's field table of a slice in LDS, over the 5 x 5 grid with its
+// edge rows and columns repeated once more: entry 7 kx + kz (kx in 0 .. 5,
+// kz in 0 .. 6) holds the field at the point (clamp(kx - 3), clamp(kz - 3)) of
+// the grid and its step to the next point in x, E(kx + 1, kz) - E(kx, kz)
+// (zero past the edge), both as (Ex, Ey): 32 bytes, two double2.  The steps
+// are the subtractions of the bilinear formula, made once per table instead of
+// once per ion, and a cell's two rows are 64 consecutive bytes.
+#define WS_QT 42 // entries per slice
+#define WS_CH 16 // slices per staged chunk of a bunch's tables (a multiple of 4)
 
-// ws_interp on a slice's padded table, with the cell anchored at the floor of
-// the grid quotient u: the cell [f, f + 1], f = floor(u), at weight u - f.
-// The reference anchors at the grid point nearer zero (trunc(u)) and weighs
-// toward the next one out; for u >= 0 that is the same cell and steps, for
-// u < 0 the same bilinear interpolant taken from its other end.  The floor and
-// the weight come from t = u + 3 in one rounding (a fused multiply-add), as
-// its integer part and v_fract_f64: the weight then carries the rounding of t
-// (at most 4.4e-16 for |u| < 3) instead of that of u.  Quotients past the grid
-// take the cell [-3, -2] or [2, 3], whose outer point repeats the edge, so the
-// field there is the edge value, as the reference's clamped partner gives.
-// All four corners are fixed offsets (0, 1, 7, 8 entries) from one address:
-// no sign tests.
+// ws_interp on a slice's table, with the cell anchored at the floor of the
+// grid quotient u: the cell [f, f + 1], f = floor(u), at weight u - f.  The
+// reference anchors at the grid point nearer zero (trunc(u)) and weighs toward
+// the next one out; for u >= 0 that is the same cell and steps, for u < 0 the
+// same bilinear interpolant taken from its other end.  The floor and the
+// weight come from t = u + 3 in one rounding (a fused multiply-add), as its
+// integer part and v_fract_f64: the weight then carries the rounding of t (at
+// most 4.4e-16 for |u| < 3) instead of that of u.  Quotients past the grid take
+// the cell [-3, -2] or [2, 3], whose outer point repeats the edge (a zero step),
+// so the field there is the edge value, as the reference's clamped partner
+// gives.  No sign tests; the four reads are consecutive.
 __device__ __forceinline__ void ws_interp_p(const double2 *t, double iax, double iaz, double xx, double yy,
                                             double &Ex, double &Ey)
 {
@@ -611,12 +631,12 @@ __device__ __forceinline__ void ws_interp_p(const double2 *t, double iax, double
   const double tx = fma(xx, iax, 3.0), tz = fma(yy, iaz, 3.0);
   const double sx = __builtin_amdgcn_fract(tx), sz = __builtin_amdgcn_fract(tz);
   const int kx = min(max((int)tx, 0), 5), kz = min(max((int)tz, 0), 5);
-  const double2 *e = t + 7 * kx + kz;
-  const double2 e1 = e[0], e2 = e[7], e3 = e[1], e4 = e[8];
-  const double y5 = e1.y + sx * (e2.y - e1.y);
-  const double y6 = e3.y + sx * (e4.y - e3.y);
-  const double x5 = e1.x + sx * (e2.x - e1.x);
-  const double x6 = e3.x + sx * (e4.x - e3.x);
+  const double2 *e = t + 2 * (7 * kx + kz);
+  const double2 e1 = e[0], d1 = e[1], e3 = e[2], d3 = e[3];
+  const double y5 = e1.y + sx * d1.y;
+  const double y6 = e3.y + sx * d3.y;
+  const double x5 = e1.x + sx * d1.x;
+  const double x6 = e3.x + sx * d3.x;
   Ey = y5 + sz * (y6 - y5);
   Ex = x5 + sz * (x6 - x5);
 }
@@ -631,7 +651,7 @@ __device__ __forceinline__ void ion_slice(const WSP &P, const double2 *qt, int c
   // the ion's kick as fused multiply-adds
 #pragma clang fp contract(fast)
   double Ex, Ey;
-  ws_interp_p(qt + c * WS_QT, P.iaAx, P.iaAz, x, y, Ex, Ey);
+  ws_interp_p(qt + c * 2 * WS_QT, P.iaAx, P.iaAz, x, y, Ex, Ey);
   xd = xd + P.dtF * Ex;
   x = x + P.deltT * xd;
   yd = yd + P.dtF * Ey;
@@ -728,11 +748,10 @@ __device__ __forceinline__ void ions_chunk(const WSP &P, const double2 *qt, int 
 // slice sums of Nri * E and the cloud sums Nri x, Nri y, Nri after the bunch,
 // each the fixed-order sum of its four waves' sums.  The bunches up to the
 // block's last creator (some lanes' ions not yet made) run with per-lane
-// predicates, the rest without.  The padded tables of a bunch are staged in
-// chunks of WS_CH slices (dynamic LDS: min(S, WS_CH) * WS_QT double2, 18.8 KB
-// at most), so that six blocks fit a CU: 78 registers a lane allow six waves
-// per SIMD, and the 1,422 blocks of the 356-bunch train are all resident at
-// once.
+// predicates, the rest without.  The tables of a bunch are staged in chunks
+// of WS_CH slices (dynamic LDS: min(S, WS_CH) * 2 WS_QT double2, 21.5 KB at
+// most), so that six blocks fit a CU: 80 registers a lane allow six waves per
+// SIMD, and the 1,422 blocks of the 356-bunch train are all resident at once.
 __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
 {
   extern __shared__ double2 qt_s[];
@@ -766,10 +785,10 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
     y = D.li[3 * GP + q];
     yd = D.li[4 * GP + q];
   }
-  // threads 0 .. 4 WS_QT - 1 stage entry r of the tables of slices sl0,
-  // sl0 + 4, ... of a chunk: a fixed grid point per thread
-  const int sl0 = tid / WS_QT, r = tid % WS_QT;
-  const int src = 12 + 5 * min(max(r / 7 - 3, -2), 2) + min(max(r % 7 - 3, -2), 2);
+  // threads 0 .. 6 WS_QT - 1 stage entry r of the tables of slices sl0,
+  // sl0 + 6, ... of a chunk: a fixed grid point and its x neighbour per thread
+  const int sl0 = tid / WS_QT, r = tid % WS_QT, gz = min(max(r % 7 - 3, -2), 2);
+  const int src = 12 + 5 * min(max(r / 7 - 3, -2), 2) + gz, src2 = 12 + 5 * min(max(r / 7 - 2, -2), 2) + gz;
   for (int ib = jmin + 1; ib < P.G; ib++) {
     const bool act = ib > jmax || j < ib;
     if (act) {
@@ -777,12 +796,16 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
       x = x + dt * xd;
       y = y + dt * yd;
     }
-    const double2 *exy = D.Exy + (size_t)ib * npts + src;
+    const double2 *exy = D.Exy + (size_t)ib * npts;
     for (int cb = 0; cb < S; cb += WS_CH) {
       const int ce = min(S, cb + WS_CH);
       __syncthreads(); // the previous chunk's tables (and bunch's sums) have been read
-      if (tid < 4 * WS_QT)
-        for (int c = cb + sl0; c < ce; c += 4) qt_s[(c - cb) * WS_QT + r] = exy[c * 25];
+      if (tid < 6 * WS_QT)
+        for (int c = cb + sl0; c < ce; c += 6) {
+          const double2 a = exy[c * 25 + src], b = exy[c * 25 + src2];
+          qt_s[((c - cb) * WS_QT + r) * 2] = a;
+          qt_s[((c - cb) * WS_QT + r) * 2 + 1] = make_double2(b.x - a.x, b.y - a.y);
+        }
       __syncthreads();
       if (ib <= jmax)
         ions_chunk<false>(P, qt_s, cb, ce, act, Nri, x, xd, y, yd, wred, lane, wv);
@@ -790,7 +813,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
         ions_chunk<true>(P, qt_s, cb, ce, true, Nri, x, xd, y, yd, wred, lane, wv);
     }
     __syncthreads();
-    // quantity-major, so that k_ws_kick reads each quantity's partials contiguously
+    // quantity-major, so that k_ws_ksum reads each quantity's partials contiguously
     const long long nbi = min(((long long)ib * npts + WS_WG - 1) / WS_WG, (long long)D.nblk_ions);
     for (int i = tid; i < NQ; i += WS_WG) {
       double v = 0.0;
@@ -800,41 +823,21 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
   }
 }
 
-// Per bunch: the blocks' partials summed in a fixed order (wave i of the
-// quantities, lanes over blocks), the slice kicks of the ion field, and the
-// cloud sums after the bunch with its own ions appended.
-__global__ __launch_bounds__(KICK_WG) void k_ws_kick(WSP P, WSD D, int slot)
+// Per bunch ib (blockIdx.x) and group of KSUM_Q quantities (blockIdx.y): the
+// blocks' partials summed in a fixed order, one quantity per wave (lane l adds
+// blocks l, l + 64, ... in turn, then the wave sum), into D.tot[ib][i].  The
+// last group of each bunch sums instead the bunch's own ions (Nri x, Nri y,
+// Nri over its S * 25 ions, four waves' sums in order) into D.tot[ib][NQ + 3 w
+// + m], which k_ws_end adds to the cloud sums after the bunch.
+#define KSUM_Q 8
+__global__ __launch_bounds__(64 * KSUM_Q) void k_ws_ksum(WSP P, WSD D)
 {
-  __shared__ double tot[2 * WS_S + 3];
-  __shared__ double red3[WS_WG / 64][3];
   const int ib = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, S = P.S, npts = S * 25;
   const int NQ = 2 * S + 3;
-  const size_t GP = (size_t)P.G * npts;
-  long long nb = ((long long)ib * npts + WS_WG - 1) / WS_WG; // blocks of k_ws_ions ahead of ib
-  if (nb > D.nblk_ions) nb = D.nblk_ions;
-  // eight quantities per wave and four blocks per lane unrolled (up to 32
-  // loads in flight per lane, 16 waves: the last bunches of a long train sum
-  // about a megabyte of partials each); each quantity's order is fixed: lane
-  // l adds blocks l, l + 64, ... in turn, then the wave sum
-  const double *tp = D.tpart + D.toff[ib];
-  for (int i0 = 8 * wv; i0 < NQ; i0 += 8 * (KICK_WG / 64)) {
-    double v[8];
-#pragma unroll
-    for (int m = 0; m < 8; m++) v[m] = 0.0;
-#pragma unroll 4
-    for (long long k = lane; k < nb; k += 64) {
-#pragma unroll
-      for (int m = 0; m < 8; m++)
-        if (i0 + m < NQ) v[m] += tp[(i0 + m) * nb + k];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-      const double t = wsum64(v[m]);
-      if (lane == 0 && i0 + m < NQ) tot[i0 + m] = t;
-    }
-  }
-  // the bunch's own ions: the first WS_WG threads, four waves' sums in order
-  if (tid < WS_WG) {
+  double *tot = D.tot + (size_t)ib * WS_TQ;
+  if (blockIdx.y == gridDim.y - 1) {
+    if (tid >= WS_WG) return;
+    const size_t GP = (size_t)P.G * npts;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     for (int p = tid; p < npts; p += WS_WG) {
       const size_t o = (size_t)ib * npts + p;
@@ -846,35 +849,54 @@ __global__ __launch_bounds__(KICK_WG) void k_ws_kick(WSP P, WSD D, int slot)
     t1 = wsum64(t1);
     t2 = wsum64(t2);
     if (lane == 0) {
-      red3[wv][0] = t0;
-      red3[wv][1] = t1;
-      red3[wv][2] = t2;
+      tot[NQ + 3 * wv] = t0;
+      tot[NQ + 3 * wv + 1] = t1;
+      tot[NQ + 3 * wv + 2] = t2;
     }
+    return;
   }
-  __syncthreads();
-  if (ib > 0) {
-    const double nel = D.nelec[ib];
-    double *xp = D.sl + (size_t)ib * 6 * S + 3 * S, *zp = xp + 2 * S;
-    for (int c = tid; c < S; c += KICK_WG) {
-      const double dQ = nel * D.npop[c] * K_E;
-      const double fjp = dQ > 0.0 ? P.factjp0 / dQ : 0.0;
-      const double sx = fjp * tot[2 * c], sy = fjp * tot[2 * c + 1];
-      xp[c] = xp[c] + P.CdeltT * sx;
-      zp[c] = zp[c] + P.CdeltT * sy;
-    }
-  }
-  if (tid < 3) {
-    double v = ib > 0 ? tot[2 * S + tid] : 0.0;
-    for (int w = 0; w < WS_WG / 64; w++) v += red3[w][tid];
-    D.ion[((size_t)slot * P.G + ib) * 3 + tid] = v;
-  }
+  const int i = blockIdx.y * KSUM_Q + wv;
+  if (ib == 0 || i >= NQ) return;
+  long long nb = ((long long)ib * npts + WS_WG - 1) / WS_WG; // blocks of k_ws_ions ahead of ib
+  if (nb > D.nblk_ions) nb = D.nblk_ions;
+  const double *tp = D.tpart + D.toff[ib] + i * nb;
+  double v = 0.0;
+#pragma unroll 4
+  for (long long k = lane; k < nb; k += 64) v += tp[k];
+  v = wsum64(v);
+  if (lane == 0) tot[i] = v;
 }
 
-// slice-averaged vertical centroid of every bucket (writeoutVCMmotionsPM2,
-// src/fbi_weakstrong.c:688-737)
-__global__ void k_ws_end(WSP P, WSD D, const int *bunch_of_bucket, int slot)
+// Blocks 0 .. nkick-1 (the ion-parallel train, nkick = G): bunch b's slice
+// kicks from the ion field sums and the cloud sums after it with its own ions
+// appended (the sums of k_ws_ksum).  The blocks after them: the slice-averaged
+// vertical centroid of every bucket (writeoutVCMmotionsPM2,
+// src/fbi_weakstrong.c:688-737), which the kicks (on the slopes) leave alone.
+__global__ void k_ws_end(WSP P, WSD D, const int *bunch_of_bucket, int slot, int nkick)
 {
-  const int kb = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x, S = P.S;
+  if ((int)blockIdx.x < nkick) {
+    const int ib = blockIdx.x, NQ = 2 * S + 3;
+    const double *tot = D.tot + (size_t)ib * WS_TQ;
+    if (ib > 0) {
+      const double nel = D.nelec[ib];
+      double *xp = D.sl + (size_t)ib * 6 * S + 3 * S, *zp = xp + 2 * S;
+      for (int c = tid; c < S; c += blockDim.x) {
+        const double dQ = nel * D.npop[c] * K_E;
+        const double fjp = dQ > 0.0 ? P.factjp0 / dQ : 0.0;
+        const double sx = fjp * tot[2 * c], sy = fjp * tot[2 * c + 1];
+        xp[c] = xp[c] + P.CdeltT * sx;
+        zp[c] = zp[c] + P.CdeltT * sy;
+      }
+    }
+    if (tid < 3) {
+      double v = ib > 0 ? tot[2 * S + tid] : 0.0;
+      for (int w = 0; w < WS_WG / 64; w++) v += tot[NQ + 3 * w + tid];
+      D.ion[((size_t)slot * P.G + ib) * 3 + tid] = v;
+    }
+    return;
+  }
+  const int kb = (blockIdx.x - nkick) * blockDim.x + tid;
   if (kb >= P.h) return;
   double z = 0.0;
   const int b = bunch_of_bucket[kb];
@@ -931,7 +953,8 @@ extern "C" void mbt_ws_destroy(mbt_ws *w)
   if (w->stream) (void)hipStreamSynchronize(w->stream); // turns of mbt_ws_track_async may still run
   void *p[] = {w->D.sl,  w->D.hist, w->D.Exy, w->D.li,     w->D.cl,     w->D.part,    w->D.psum,
                w->D.done, w->D.cm,  w->D.mon,  w->D.ion, w->D.zcm,    w->d_bucket, w->d_fill,    w->d_npop,
-               w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff};
+               w->d_nofe, w->d_bob, w->d_Ibunch, w->d_nelec, w->d_convion, w->D.tpart, w->d_dtime, w->d_toff,
+               w->D.tot};
   for (size_t i = 0; i < sizeof p / sizeof p[0]; i++)
     if (p[i]) (void)hipFree(p[i]);
   if (w->ev)
@@ -1124,6 +1147,7 @@ extern "C" int mbt_ws_create(const mbt_config *c, int device, int flags, mbt_ws 
     w->serial = (flags & MBT_WS_SERIAL_TRAIN) || G < 2 || ntp * (long long)sizeof(double) > WS_TRAIN_PARTIAL_MAX;
     if (!w->serial && P.fbii) {
       WA(w->D.tpart, ntp);
+      WA(w->D.tot, (size_t)G * WS_TQ);
       WA(w->d_toff, G);
       WUP(w->d_toff, toff, G);
       CK(hipStreamSynchronize(w->stream));
@@ -1194,22 +1218,33 @@ static int ws_enqueue(mbt_ws *w, long n)
 {
   const int G = w->P.G, H = w->P.h;
   const size_t lds2 = sizeof(double) * 2 * H;
+  // phase 1: a block per bunch, and with FBII one thread per grid point of
+  // every slice of every bunch (for the fields, then the ions)
+  const long long np = (long long)G * w->P.S * 25;
+  const int nbp = (int)((np + WS_WG - 1) / WS_WG), nb1 = w->P.fbii && nbp > G ? nbp : G;
+  const int nkick = w->P.fbii && !w->serial ? G : 0;
   CK(hipMemsetAsync(w->D.mon, 0, sizeof(double) * 18 * G * n, w->stream));
   CK(hipMemsetAsync(w->D.ion, 0, sizeof(double) * 3 * G * n, w->stream));
   for (long t = 0; t < n; t++) {
     const long long rev = w->rev;
     const int monitor = rev % w->cfg.NrevMon == 0;
-    hipLaunchKernelGGL(k_ws_phase1, dim3(G), dim3(WS_WG), 0, w->stream, w->P, w->D, rev, (int)t, monitor);
+    hipLaunchKernelGGL(k_ws_phase1, dim3(nb1), dim3(WS_WG), 0, w->stream, w->P, w->D, rev, (int)t, monitor);
     CK(hipGetLastError());
+    if (w->P.fbii) {
+      // the ions start from the slices' positions before the betatron rotation
+      hipLaunchKernelGGL(k_ws_create, dim3(nbp), dim3(WS_WG), 0, w->stream, w->P, w->D);
+      CK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_ws_phase2, dim3(G), dim3(WS_WG), lds2, w->stream, w->P, w->D, rev);
     CK(hipGetLastError());
     if (w->timing && w->P.fbii) CK(hipEventRecord(w->ev[2 * t], w->stream));
     if (w->P.fbii && !w->serial) {
-      hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * WS_QT * min(w->P.S, WS_CH),
+      hipLaunchKernelGGL(k_ws_ions, dim3(w->D.nblk_ions), dim3(WS_WG), sizeof(double2) * 2 * WS_QT * min(w->P.S, WS_CH),
                          w->stream, w->P, w->D);
       CK(hipGetLastError());
       if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
-      hipLaunchKernelGGL(k_ws_kick, dim3(G), dim3(KICK_WG), 0, w->stream, w->P, w->D, (int)t);
+      hipLaunchKernelGGL(k_ws_ksum, dim3(G, (2 * w->P.S + 3 + KSUM_Q - 1) / KSUM_Q + 1), dim3(64 * KSUM_Q), 0,
+                         w->stream, w->P, w->D);
       CK(hipGetLastError());
     } else if (w->P.fbii) {
       hipLaunchKernelGGL(k_ws_cloud_reset, dim3(1), dim3(WS_WG), 0, w->stream, w->P, w->D, (int)t);
@@ -1224,7 +1259,8 @@ static int ws_enqueue(mbt_ws *w, long n)
       }
       if (w->timing) CK(hipEventRecord(w->ev[2 * t + 1], w->stream));
     }
-    hipLaunchKernelGGL(k_ws_end, dim3((H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t);
+    hipLaunchKernelGGL(k_ws_end, dim3(nkick + (H + 255) / 256), dim3(256), 0, w->stream, w->P, w->D, w->d_bob, (int)t,
+                       nkick);
     CK(hipGetLastError());
     w->rev++;
   }
