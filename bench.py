@@ -38,8 +38,10 @@ small ring (64 bunches x 1,000 macroparticles, QE on) in FP64 and mixed
 precision (the lone ring's RW sum as the FP32 GEMM), with the mixed run's
 errors and the growth rates of a resistive-wall instability in both.  `fbii`
 is the weak-strong fast beam-ion workload (356-bunch train in h = 416), in
-ion-slice interactions/s, with the ion-train kernel's FP64 roofline and the
-reference's own functions timed beside it.
+ion-slice interactions/s over one 100-turn call (the first turns after a host
+synchronisation run slower, 4.0 falling to 3.4 ms in a kernel trace, so a
+short call understates a long run's rate), with the ion-train kernel's FP64
+roofline and the reference's own functions timed beside it.
 """
 import argparse
 import json
@@ -484,7 +486,7 @@ def main():
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production line (profiling of the other lines only)")
     ap.add_argument("--no-fbii", action="store_true", help="skip the fast beam-ion line")
-    ap.add_argument("--fbii-steps", type=int, default=20)
+    ap.add_argument("--fbii-steps", type=int, default=100)
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()

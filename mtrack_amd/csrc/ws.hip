@@ -604,11 +604,14 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
 
 // The ion train's field table of a slice in LDS, over the 5 x 5 grid with its
 // edge rows and columns repeated once more: entry 7 kx + kz (kx in 0 .. 5,
-// kz in 0 .. 6) holds the field at the point (clamp(kx - 3), clamp(kz - 3)) of
-// the grid and its step to the next point in x, E(kx + 1, kz) - E(kx, kz)
-// (zero past the edge), both as (Ex, Ey): 32 bytes, two double2.  The steps
-// are the subtractions of the bilinear formula, made once per table instead of
-// once per ion, and a cell's two rows are 64 consecutive bytes.
+// kz in 0 .. 6) of the first WS_QT double2 holds the field (Ex, Ey) at the
+// point (clamp(kx - 3), clamp(kz - 3)) of the grid, the same entry of the next
+// WS_QT its step to the next point in x, E(kx + 1, kz) - E(kx, kz) (zero past
+// the edge).  The steps are the subtractions of the bilinear formula, made once
+// per table instead of once per ion.  Two arrays of 16-byte entries rather than
+// 32-byte (field, step) pairs: the lanes of one read then spread over all
+// sixteen 16-byte bank groups of the LDS, not eight (measured: a sixth of the
+// bank-conflict cycles, SQ_LDS_BANK_CONFLICT).
 #define WS_QT 42 // entries per slice
 #define WS_CH 16 // slices per staged chunk of a bunch's tables (a multiple of 4)
 
@@ -622,7 +625,7 @@ __global__ __launch_bounds__(WS_WG) void k_ws_train(WSP P, WSD D, int ib, long l
 // most 4.4e-16 for |u| < 3) instead of that of u.  Quotients past the grid take
 // the cell [-3, -2] or [2, 3], whose outer point repeats the edge (a zero step),
 // so the field there is the edge value, as the reference's clamped partner
-// gives.  No sign tests; the four reads are consecutive.
+// gives.  No sign tests; the four reads are fixed offsets from one address.
 __device__ __forceinline__ void ws_interp_p(const double2 *t, double iax, double iaz, double xx, double yy,
                                             double &Ex, double &Ey)
 {
@@ -631,8 +634,8 @@ __device__ __forceinline__ void ws_interp_p(const double2 *t, double iax, double
   const double tx = fma(xx, iax, 3.0), tz = fma(yy, iaz, 3.0);
   const double sx = __builtin_amdgcn_fract(tx), sz = __builtin_amdgcn_fract(tz);
   const int kx = min(max((int)tx, 0), 5), kz = min(max((int)tz, 0), 5);
-  const double2 *e = t + 2 * (7 * kx + kz);
-  const double2 e1 = e[0], d1 = e[1], e3 = e[2], d3 = e[3];
+  const double2 *e = t + 7 * kx + kz;
+  const double2 e1 = e[0], e3 = e[1], d1 = e[WS_QT], d3 = e[WS_QT + 1];
   const double y5 = e1.y + sx * d1.y;
   const double y6 = e3.y + sx * d3.y;
   const double x5 = e1.x + sx * d1.x;
@@ -803,8 +806,8 @@ __global__ __launch_bounds__(WS_WG) void k_ws_ions(WSP P, WSD D)
       if (tid < 6 * WS_QT)
         for (int c = cb + sl0; c < ce; c += 6) {
           const double2 a = exy[c * 25 + src], b = exy[c * 25 + src2];
-          qt_s[((c - cb) * WS_QT + r) * 2] = a;
-          qt_s[((c - cb) * WS_QT + r) * 2 + 1] = make_double2(b.x - a.x, b.y - a.y);
+          qt_s[(c - cb) * 2 * WS_QT + r] = a;
+          qt_s[(c - cb) * 2 * WS_QT + WS_QT + r] = make_double2(b.x - a.x, b.y - a.y);
         }
       __syncthreads();
       if (ib <= jmax)
