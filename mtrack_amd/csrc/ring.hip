@@ -323,6 +323,19 @@ __device__ __noinline__ double2 sincos_lib(double x)
   return r;
 }
 
+// fma(a, b, c) as the three-address v_fma_f64.  The compiler turns a
+// polynomial step fma(z, p, coefficient) into the two-address v_fmac_f64,
+// whose addend is also its result, and so copies the loop-invariant
+// coefficient into a fresh register before every step (v_mov_b64 + v_fmac_f64,
+// measured in the gfx950 assembly of k_turn); the same IEEE fused multiply-add
+// in one instruction.  Register operands only, no memory access.
+__device__ __forceinline__ double fma3(double a, double b, double c)
+{
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 __device__ __forceinline__ void sincos_cw(double x, double *s, double *c)
 {
   if (!(fabs(x) < 823549.6)) {
@@ -336,21 +349,21 @@ __device__ __forceinline__ void sincos_cw(double x, double *s, double *c)
   r = fma(-k, 6.07710050630396597660e-11, r);
   r = fma(-k, 2.02226624879595063154e-21, r);
   const double z = r * r;
-  double ps = fma(z, 2.8114572543455206e-15, -7.647163731819816e-13);
-  ps = fma(z, ps, 1.6059043836821613e-10);
-  ps = fma(z, ps, -2.505210838544172e-08);
-  ps = fma(z, ps, 2.7557319223985893e-06);
-  ps = fma(z, ps, -0.0001984126984126984);
-  ps = fma(z, ps, 0.008333333333333333);
-  ps = fma(z, ps, -0.16666666666666666);
+  double ps = fma3(z, 2.8114572543455206e-15, -7.647163731819816e-13);
+  ps = fma3(z, ps, 1.6059043836821613e-10);
+  ps = fma3(z, ps, -2.505210838544172e-08);
+  ps = fma3(z, ps, 2.7557319223985893e-06);
+  ps = fma3(z, ps, -0.0001984126984126984);
+  ps = fma3(z, ps, 0.008333333333333333);
+  ps = fma3(z, ps, -0.16666666666666666);
   const double sn = fma(r * z, ps, r);
-  double pc = fma(z, -1.5619206968586225e-16, 4.779477332387385e-14);
-  pc = fma(z, pc, -1.1470745597729725e-11);
-  pc = fma(z, pc, 2.08767569878681e-09);
-  pc = fma(z, pc, -2.755731922398589e-07);
-  pc = fma(z, pc, 2.48015873015873e-05);
-  pc = fma(z, pc, -0.001388888888888889);
-  pc = fma(z, pc, 0.041666666666666664);
+  double pc = fma3(z, -1.5619206968586225e-16, 4.779477332387385e-14);
+  pc = fma3(z, pc, -1.1470745597729725e-11);
+  pc = fma3(z, pc, 2.08767569878681e-09);
+  pc = fma3(z, pc, -2.755731922398589e-07);
+  pc = fma3(z, pc, 2.48015873015873e-05);
+  pc = fma3(z, pc, -0.001388888888888889);
+  pc = fma3(z, pc, 0.041666666666666664);
   const double hz = 0.5 * z;
   const double w = 1.0 - hz;
   const double cs = w + (((1.0 - w) - hz) + (z * z) * pc);
@@ -2212,9 +2225,12 @@ __global__ __launch_bounds__(WG) void k_stream(Coords C, long long n, unsigned l
 // footprints search: candidates are allocated while the device has room for
 // one more (at most PLACE_CANDIDATES), each timed with k_stream, the first at
 // PLACE_GOOD_GBS or above kept at once, else the fastest; the others are freed.
+// The bar sits near the top of the fast placements' range, so that a fast but
+// not the fastest first allocation (6.0 TB/s) is not kept without a look at
+// the next ones.
 #define PLACE_SEARCH_MIN (16ULL << 30)
 #define PLACE_CANDIDATES 3
-#define PLACE_GOOD_GBS 6000.0
+#define PLACE_GOOD_GBS 6300.0
 
 static int alloc_six(Coords &c, long long n)
 {
