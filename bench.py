@@ -272,12 +272,11 @@ def one_ring_at_a_time(nrings, seed, steps, warmup):
 
 
 def placement(achieved, ceiling):
-    """k_turn's rate against a plain in-place update of the same six coordinate
-    arrays (mbt_ring_stream_ceiling): the ceiling the arrays' placement in HBM
-    sets for this traffic (it differs between allocations, 5.2-5.4 or 6.1-6.3
-    TB/s on the devices measured)."""
+    """k_turn's rate against a plain in-place update of the same coordinate
+    tiles (mbt_ring_stream_ceiling): the ceiling the tiles' placement in HBM
+    sets for this traffic (it differs between allocations)."""
     return {"GB/s": ceiling, "k_turn_frac": achieved / ceiling,
-            "what": "plain in-place update of the same six coordinate arrays, bit-preserving, 96 B per slot"}
+            "what": "plain in-place update of the same coordinate tiles, bit-preserving, 96 B per slot"}
 
 
 def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):

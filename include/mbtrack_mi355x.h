@@ -297,10 +297,11 @@ int mbt_ring_get_chunk(const mbt_ring *r);
  * not tried.  Call after mbt_ring_prepare. */
 int mbt_ring_autotune(mbt_ring *r, int *best_items, double *ms);
 /* No counterpart in the reference (a measurement aid): the rate in GB/s of a
- * plain in-place update of the ring's own six coordinate arrays (96 bytes per
- * slot, every bit written back unchanged, so the ring's state is untouched),
- * averaged over reps passes after one untimed pass -- the ceiling the arrays'
- * placement in device memory sets for the per-macroparticle pass's traffic. */
+ * plain in-place update of the ring's own coordinates (six FP64 values, 96
+ * bytes per slot, every bit written back unchanged, so the ring's state is
+ * untouched), averaged over reps passes after one untimed pass -- the ceiling
+ * their layout and placement in device memory set for the per-macroparticle
+ * pass's traffic. */
 int mbt_ring_stream_ceiling(mbt_ring *r, int reps, double *gbps);
 
 /* ---- ensembles of independent rings on one GPU ------------------------------ */

@@ -56,7 +56,7 @@
 // tail costs more than the one-pass sum)
 #define RW_SPLIT_MIN_MP (1LL << 22)
 // coordinate footprints from which the set-up reserves and releases the whole
-// footprint before allocating the six arrays
+// footprint before allocating the tiled slab
 #define COORD_RESERVE_MIN (1ULL << 30)
 // consecutive taps per RW tile of k_turn's leading blocks, and its LDS budget
 // (the slots of the tile's bunches and taps; k_turn's dynamic LDS grows to it)
@@ -2214,11 +2214,9 @@ static int setup_rw_split(mbt_ring *r)
   return MBT_OK;
 }
 
-// Plain in-place update of the six coordinate arrays, every bit written back
-// (xor with a zero the compiler cannot see): the rate the arrays' placement
-// allows for k_turn's loads and stores, without its work.  2,048 slots per block.
-// One tile per work-group: the bit-preserving in-place update of its six runs
-// (the turn pass's traffic without its work).
+// Plain in-place update of the coordinate tiles, every bit written back (xor
+// with a zero the compiler cannot see): the rate the slab's placement allows
+// for k_turn's loads and stores, without its work.  One tile per work-group.
 __global__ __launch_bounds__(WG) void k_stream(Coords C, unsigned long long zero)
 {
   double *base = C.s + ((long long)blockIdx.x * 6 << TILE_SHIFT);
