@@ -810,7 +810,7 @@ struct RedOut {
 __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr, const double *partial,
                                                const int *bunch_wg0, const int *bunch_ring, const int *bucket,
                                                HistPtr H, RedOut R, const TurnState *__restrict__ tsr, int phase,
-                                               const int *lr_planes)
+                                               const int *lr_planes, int dip)
 {
   extern __shared__ double sm[];
   const int b = blockIdx.x, tid = threadIdx.x;
@@ -922,22 +922,32 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
     return;
   }
   // ---- histograms: clamped (selffield) and dropped (LR resonators)
+  // dip = 0: the pass before (a turn pass without dipole histograms) left the
+  // dipole sums at the zeros the last clear wrote, so they are neither read
+  // nor cleared here (the outputs get the same zeros)
   const bool bad = H.bad[b] != 0;
   const size_t lrrow = (size_t)(P.bucket_off + kb) * Nc;
   for (int i = tid; i < Nc; i += WG) {
     size_t g = (size_t)b * Nc + i;
     double cnt = (double)H.cnt[g];
-    double sx = bad ? NAN : fx_to_double(H.sx[2 * g], H.sx[2 * g + 1]);
-    double sz = bad ? NAN : fx_to_double(H.sz[2 * g], H.sz[2 * g + 1]);
+    unsigned long long xl = 0, xh = 0, zl = 0, zh = 0;
+    if (dip) {
+      xl = H.sx[2 * g];
+      xh = H.sx[2 * g + 1];
+      zl = H.sz[2 * g];
+      zh = H.sz[2 * g + 1];
+    }
+    double sx = bad ? NAN : fx_to_double(xl, xh);
+    double sz = bad ? NAN : fx_to_double(zl, zh);
     if (lr_planes[0]) R.fnp_ring[lrrow + i] = cnt;
     if (lr_planes[1]) R.fnp_ring[R.plane_stride + lrrow + i] = sx;
     if (lr_planes[2]) R.fnp_ring[2 * R.plane_stride + lrrow + i] = sz;
     // clamped: out-of-range particles join the edge bins
-    unsigned long long xl = H.sx[2 * g], xh = H.sx[2 * g + 1], zl = H.sz[2 * g], zh = H.sz[2 * g + 1];
     unsigned int c = H.cnt[g];
     for (int tt = 0; tt < 2; tt++) {
       if (!((tt == 0 && i == 0) || (tt == 1 && i == Nc - 1))) continue;
       c += H.tcnt[2 * b + tt];
+      if (!dip) continue;
       unsigned long long al = H.tsx[4 * b + 2 * tt], ah = H.tsx[4 * b + 2 * tt + 1];
       unsigned long long nl = xl + al;
       xh = xh + ah + (nl < xl ? 1ULL : 0ULL);
@@ -975,13 +985,17 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
   for (int i = tid; i < Nc; i += WG) {
     size_t g = (size_t)b * Nc + i;
     H.cnt[g] = 0;
-    H.sx[2 * g] = H.sx[2 * g + 1] = 0;
-    H.sz[2 * g] = H.sz[2 * g + 1] = 0;
+    if (dip) {
+      H.sx[2 * g] = H.sx[2 * g + 1] = 0;
+      H.sz[2 * g] = H.sz[2 * g + 1] = 0;
+    }
   }
   if (tid < 2) {
     H.tcnt[2 * b + tid] = 0;
-    H.tsx[4 * b + 2 * tid] = H.tsx[4 * b + 2 * tid + 1] = 0;
-    H.tsz[4 * b + 2 * tid] = H.tsz[4 * b + 2 * tid + 1] = 0;
+    if (dip) {
+      H.tsx[4 * b + 2 * tid] = H.tsx[4 * b + 2 * tid + 1] = 0;
+      H.tsz[4 * b + 2 * tid] = H.tsz[4 * b + 2 * tid + 1] = 0;
+    }
   }
   if (tid == 0) H.bad[b] = 0;
   const int icmin = icmin_s, icmax = icmax_s;
@@ -1038,7 +1052,9 @@ __global__ __launch_bounds__(WG) void k_reduce(const DevParams *__restrict__ Pr,
         }
         val = 1 * Gn;
       }
-      (pl == 1 ? R.sfV : R.sfH)[(size_t)b * Nc + i] = -factG * val;
+      // a plane without a kick keeps the +0.0 of its allocation (its value
+      // here, -factG * 0 with factG <= 0), so its table is not rewritten
+      if (pl == 1 ? P.kickV : P.kickH) (pl == 1 ? R.sfV : R.sfH)[(size_t)b * Nc + i] = -factG * val;
     }
   }
 }
@@ -2967,8 +2983,11 @@ static int launch_turn_pass(mbt_ring *r, int mode)
 
 static int launch_reduce(mbt_ring *r, int phase)
 {
+  // the statistics passes (phases 0, 1, 3) always make the dipole sums; a turn
+  // pass only when a ring's kicks or LR resonators need them
+  const int dip = phase != 2 || r->any_dip;
   hipLaunchKernelGGL(k_reduce, dim3(r->G), dim3(WG), r->lds_red, r->stream, r->P, r->partial, r->bunch_wg0,
-                     r->bunch_ring, r->bucket, r->Hs, r->Ro, r->ts, phase, r->lr_planes);
+                     r->bunch_ring, r->bucket, r->Hs, r->Ro, r->ts, phase, r->lr_planes, dip);
   CK(hipGetLastError());
   return MBT_OK;
 }
