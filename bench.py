@@ -33,7 +33,10 @@ in mixed precision (FP32 wake convolutions, FP32 hi/lo MFMA resistive-wall
 GEMM) with its errors against the FP64 run; `batched.one_ring_at_a_time_fp64`
 tracks the same rings one after another through the single-ring FP64 path at
 each ring's steady rate (the fixed cost of a call excluded), and
-`vs_one_ring_at_a_time` is the batched forms' ratio to it.  `c1` is the
+`vs_one_ring_at_a_time` is the batched forms' ratio to it, steady rate to
+steady rate (`batched.steady`: a 60-turn call minus a 20-turn call, the same
+outputs), with `vs_one_ring_at_a_time_per_call` the ratio of 20-turn calls,
+fixed costs included on both sides.  `c1` is the
 small ring (64 bunches x 1,000 macroparticles, QE on) in FP64 and mixed
 precision (the lone ring's RW sum as the FP32 GEMM), with the mixed run's
 errors and the growth rates of a resistive-wall instability in both.  `fbii`
@@ -238,7 +241,8 @@ def one_ring_at_a_time(nrings, seed, steps, warmup):
     """The batched workload's rings tracked one after another, each alone through
     the plain FP64 single-ring path (its own default decomposition and graphs),
     at each ring's steady rate: the ring's `steps`-turn and 3 x `steps`-turn
-    calls are timed (both graphs instantiated first) and the difference, 2 x
+    calls are timed (both graphs instantiated first, at the same phase of the
+    monitor interval as the timed calls) and the difference, 2 x
     `steps` turns without the fixed cost of a call (synchronisation, output
     copies, the first turns after an idle host phase), is summed over the rings.
     A user tracking thousands of turns per ring pays that fixed cost once."""
@@ -252,13 +256,25 @@ def one_ring_at_a_time(nrings, seed, steps, warmup):
             g.prepare()
             if warmup > 0:
                 g.track(warmup)
-            g.track(steps, want_amp=False)  # untimed: the graphs of both block lengths
-            g.track(3 * steps, want_amp=False)
-            t0 = time.perf_counter()
-            g.track(steps, want_amp=False)
-            t1 = time.perf_counter()
-            g.track(3 * steps, want_amp=False)
-            t2 = time.perf_counter()
+            # untimed: the graphs of both block lengths, then a filler call back
+            # to the same phase of the monitor interval, so that the timed calls
+            # replay graphs already built (turn graphs are cached by block length
+            # and the offset of the block's monitor turn)
+            mon_every = cfg.geti("NrevMon")
+            for _ in range(2):
+                g.track(steps, want_amp=False)
+                g.track(3 * steps, want_amp=False)
+                fill = (-4 * steps) % mon_every
+                if fill:
+                    g.track(fill, want_amp=False)
+                b0 = g.graph_builds()
+                t0 = time.perf_counter()
+                g.track(steps, want_amp=False)
+                t1 = time.perf_counter()
+                g.track(3 * steps, want_amp=False)
+                t2 = time.perf_counter()
+                if g.graph_builds() == b0:
+                    break
             total_s += (t2 - t1) - (t1 - t0)
             short_s += t1 - t0
             total_mp += g.Nb * g.Np
@@ -323,6 +339,31 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
         bytes_turn = ens.bytes_per_turn()
         ens.set_timing(False)
         ceiling = ens.stream_ceiling(5)
+        # steady rate, taken as one_ring_at_a_time takes its rings': a
+        # 3 x steps-turn call minus a steps-turn call, the same outputs (no
+        # ampinv).  Turn graphs are cached by block length and the offset of the
+        # block's monitor turn, so the untimed pair instantiates both and an
+        # untimed filler call brings the timed pair back to the same phase of
+        # the monitor interval (checked: no graph built while timed)
+        tracked = warmup + 3 * steps
+        mon_every = cfgs[0].geti("NrevMon")
+        steady = None
+        for _ in range(2):
+            ens.track(steps, want_amp=False)
+            ens.track(3 * steps, want_amp=False)
+            fill = (-4 * steps) % mon_every
+            if fill:
+                ens.track(fill, want_amp=False)
+            b0 = ens.graph_builds()
+            s0 = time.perf_counter()
+            ens.track(steps, want_amp=False)
+            s1 = time.perf_counter()
+            ens.track(3 * steps, want_amp=False)
+            s2 = time.perf_counter()
+            tracked += 8 * steps + fill
+            if ens.graph_builds() == b0:
+                steady = (s2 - s1) - (s1 - s0)
+                break
         growth = None
         if growth_turns > 0:
             # per-ring growth rates of the centroid action over further turns,
@@ -332,7 +373,7 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
             ens.set_timing(False)
             mon = cfgs[0].geti("NrevMon")
             rows, tmon = [], []
-            start = warmup + 3 * steps  # turns tracked so far
+            start = tracked  # turns tracked so far
             for blk in range(growth_turns // mon):
                 if blk % 10 == 0:
                     print("growth-rate turns: %d of %d" % (blk * mon, growth_turns), file=sys.stderr, flush=True)
@@ -368,6 +409,10 @@ def batched(steps, warmup, nrings, seed, chunk=0, mixed=False, growth_turns=0):
                          "traffic_source": TRAFFIC_NOTE % os.path.relpath(TRAFFIC_FILE_BATCHED, ROOT)},
             "instrumented": {"note": "same number of turns launched eagerly, HIP events around every kernel class",
                              "ms_per_step": 1e3 * (e1 - e0) / steps, "value": N * steps / (e1 - e0)},
+            "steady": ({"note": "a %d-turn call minus a %d-turn call, statistics and RF outputs: the fixed cost of "
+                                "a call excluded, as in one_ring_at_a_time_fp64" % (3 * steps, steps),
+                        "ms_per_step": 1e3 * steady / (2 * steps), "value": N * 2 * steps / steady}
+                       if steady else None),
             "placement_ceiling": dict(placement(achieved, ceiling), search=search), "create_s": create_s,
             **({"growth_rates": growth} if growth else {})}, last_stats
 
@@ -584,9 +629,14 @@ def main():
             if not a.no_one_at_a_time:
                 single = one_ring_at_a_time(a.batched_rings, a.batched_seed, a.batched_steps, 3)
                 out["batched"]["one_ring_at_a_time_fp64"] = single
-                out["batched"]["vs_one_ring_at_a_time"] = out["batched"]["value"] / single["value"]
-                if "mixed" in out["batched"]:
-                    out["batched"]["mixed"]["vs_one_ring_at_a_time"] = out["batched"]["mixed"]["value"] / single["value"]
+                # like for like: steady rates (fixed cost of a call excluded, the
+                # same outputs) on both sides; the one-call ratio beside it
+                # compares the timed steps-turn call with the rings' own
+                # steps-turn calls, fixed costs included on both sides
+                for form in [out["batched"]] + ([out["batched"]["mixed"]] if "mixed" in out["batched"] else []):
+                    st = form.get("steady")
+                    form["vs_one_ring_at_a_time"] = (st["value"] if st else form["value"]) / single["value"]
+                    form["vs_one_ring_at_a_time_per_call"] = form["value"] / single["per_call_value"]
         except Exception as e:  # reported beside the FP64 line, never replaces it
             out["batched"] = {"value": None, "error": str(e)[:300]}
     if not a.no_fbii:
