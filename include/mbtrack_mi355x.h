@@ -222,7 +222,13 @@ typedef struct mbt_ring mbt_ring;
 
 int mbt_ring_create(const mbt_config *cfg, int device, int flags, mbt_ring **out);
 void mbt_ring_destroy(mbt_ring *r);
-/* Replace the coordinates of filled bunch ordinal ib (SoA layout as mbt_generate_bunch). */
+/* Replace / read the coordinates of filled bunch ordinal ib (SoA layout as
+ * mbt_generate_bunch), as the bunch stands at the end of a turn: the device
+ * applies a turn's resistive-wall kick (src/transform_weak.c:780-833) at the
+ * start of the next pass, so _get_bunch adds the kick still pending and
+ * _set_bunch drops the bunch's pending kick.  Between tracked turns the next
+ * turn's wake tables come from the slice histograms of the coordinates that
+ * were tracked; replace bunches before mbt_ring_prepare. */
 int mbt_ring_set_bunch(mbt_ring *r, int ib, const double *soa6);
 int mbt_ring_get_bunch(const mbt_ring *r, int ib, double *soa6);
 /* Finishes set-up after coordinates are final: initial statistics, LR-resonator

@@ -1,7 +1,8 @@
 // The multi-bunch tracking turn on one MI355X (gfx950).
 //
-// Every bunch is resident in HBM as six FP64 coordinate arrays (tau, delta, x,
-// x', z, z'), bunches back to back.  The unit is an ensemble of independent
+// Every bunch is resident in HBM as six FP64 coordinates (tau, delta, x, x', z,
+// z') in tiles of 4,096 slots: a tile holds one run of 4,096 values per
+// coordinate, each bunch starts on a tile and bunches follow one another.  The unit is an ensemble of independent
 // rings (one ring is an ensemble of one): per-ring constants sit in a DevParams
 // table indexed through bunch -> ring, so every launch covers all rings and
 // rings of uneven bunch and macroparticle counts share each kernel.  One turn
@@ -1799,6 +1800,7 @@ struct mbt_ring {
   double *sfL, *sfV, *sfH, *lr_wake, *fnp_ring, *lrc;
   double *Gf;
   double *pending, *rwhist, *sqrt_tt, *rsqrt_tt, *Ibunch, *bunch_Ib;
+  double *peek; // one bunch's six coordinates in SoA order, for mbt_ensemble_get_bunch
   double *stats, *amp, *rfbuf;
   double *fnp_out, *dipV_out, *dipH_out;
   int *trash_tot, *trash_pend;
@@ -1993,7 +1995,7 @@ extern "C" void mbt_ring_destroy(mbt_ring *r)
   void *ptrs[] = {r->C.s, r->wg_first, r->wg_end, r->bunch_first,
                   r->wg_bunch, r->bunch_wg0, r->bucket, r->bunch_ring, r->bucket_ring, r->np, r->nfFill,
                   r->lr_planes, r->P, r->partial, r->shift, r->Hs.cnt, r->Hs.sx, r->Hs.sz, r->Hs.tcnt, r->Hs.tsx,
-                  r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending,
+                  r->Hs.tsz, r->Hs.bad, r->sfL, r->sfV, r->sfH, r->lr_wake, r->fnp_ring, r->lrc, r->Gf, r->pending, r->peek,
                   r->rwhist, r->sqrt_tt, r->rsqrt_tt, r->Ibunch, r->bunch_Ib, r->stats, r->amp, r->rfbuf, r->fnp_out,
                   r->dipV_out, r->dipH_out, r->trash_tot, r->trash_pend, r->ts, r->rw_tiles, r->rw_part, r->ring_done, r->lc, r->lr_first, r->phasor_end, r->fb_hist,
                   r->rwg_col_ring, r->rwg_bucket_bunch, r->rwg_c, r->rwg_part};
@@ -2600,6 +2602,12 @@ static int create_rings(const mbt_config *const *cfgs, int R, int device, int fl
   DA(r->fnp_ring, (size_t)3 * H * NcA);
   DA(r->Gf, (size_t)R * 3 * NcA);
   DA(r->pending, (size_t)G * 2);
+  {
+    long long npmax = 1;
+    for (int k = 0; k < R; k++)
+      if (r->rh[k].cfg.Np > npmax) npmax = r->rh[k].cfg.Np;
+    DA(r->peek, (size_t)6 * npmax);
+  }
   DA(r->rwhist, rw_total > 0 ? rw_total : 2);
   DA(r->sqrt_tt, st_total > 0 ? st_total : 1);
   DA(r->rsqrt_tt, st_total > 0 ? st_total : 1);
@@ -2878,13 +2886,14 @@ extern "C" int mbt_ensemble_create(const mbt_config *const *cfgs, int nrings, in
   return create_rings(cfgs, nrings, device, flags, out);
 }
 
-static long long bunch_first_host(const mbt_ring *r, int g, int *np)
+static long long bunch_first_host(const mbt_ring *r, int g, int *np, int *ring = NULL)
 {
   long long pos = 0;
   for (int k = 0; k < r->R; k++) {
     const mbt_config *c = &r->rh[k].cfg;
     if (g < c->Nbunch) {
       *np = c->Np;
+      if (ring) *ring = k;
       return pos + (long long)g * padded_np(c->Np);
     }
     g -= c->Nbunch;
@@ -2892,6 +2901,22 @@ static long long bunch_first_host(const mbt_ring *r, int g, int *np)
   }
   *np = 0;
   return -1;
+}
+
+// One bunch's coordinates out of the tiles in SoA order, with the resistive-wall
+// kick of the last turn (src/transform_weak.c:780-833), which the next pass
+// applies first, added the way that pass adds it: the bunch as it stands at the
+// end of the turn.  Grid (ceil(Np / 256), 6).
+__global__ __launch_bounds__(256) void k_peek(Coords C, long long first, int Np, const double *pending, int g,
+                                              int rwh, int rwv, double *out)
+{
+  const int c = blockIdx.y;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= Np) return;
+  double v = C.s[coord_idx(first + j, c)];
+  if (c == 3 && rwh) v += pending[2 * g];
+  if (c == 5 && rwv) v += pending[2 * g + 1];
+  out[(size_t)c * Np + j] = v;
 }
 
 extern "C" int mbt_ensemble_set_bunch(mbt_ring *r, int g, const double *s)
@@ -2910,6 +2935,9 @@ extern "C" int mbt_ensemble_set_bunch(mbt_ring *r, int g, const double *s)
       CK(hipMemcpyAsync(r->C.s + coord_idx(first + (nft << TILE_SHIFT), c), s + (size_t)c * Np + (nft << TILE_SHIFT),
                         rem * sizeof(double), hipMemcpyHostToDevice, r->stream));
   }
+  // the coordinates handed in are the bunch at the end of a turn: no kick of the
+  // last turn is left to apply to them
+  CK(hipMemsetAsync(r->pending + 2 * (size_t)g, 0, 2 * sizeof(double), r->stream));
   CK(hipStreamSynchronize(r->stream));
   return MBT_OK;
 }
@@ -2918,18 +2946,16 @@ extern "C" int mbt_ensemble_get_bunch(const mbt_ring *r, int g, double *s)
 {
   if (g < 0 || g >= r->G) return MBT_EINVAL;
   CK(hipSetDevice(r->device));
+  int Np, ring;
+  const long long first = bunch_first_host(r, g, &Np, &ring);
+  if (Np <= 0) return MBT_OK;
+  const DevParams &P = r->Ph[ring];
+  (void)hipGetLastError(); // the launch check below must not report an earlier call's error
+  hipLaunchKernelGGL(k_peek, dim3((unsigned)((Np + 255) / 256), 6), dim3(256), 0, r->stream, r->C, first, Np,
+                     (const double *)r->pending, g, P.rw && P.track_h, P.rw && P.track_v, r->peek);
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(s, r->peek, (size_t)6 * Np * sizeof(double), hipMemcpyDeviceToHost, r->stream));
   CK(hipStreamSynchronize(r->stream));
-  int Np;
-  const long long first = bunch_first_host(r, g, &Np);
-  const long long nft = Np >> TILE_SHIFT, rem = Np & (TILE - 1);
-  for (int c = 0; c < 6; c++) {
-    if (nft)
-      CK(hipMemcpy2D(s + (size_t)c * Np, TILE * sizeof(double), r->C.s + coord_idx(first, c),
-                     6 * TILE * sizeof(double), TILE * sizeof(double), nft, hipMemcpyDeviceToHost));
-    if (rem)
-      CK(hipMemcpy(s + (size_t)c * Np + (nft << TILE_SHIFT), r->C.s + coord_idx(first + (nft << TILE_SHIFT), c),
-                   rem * sizeof(double), hipMemcpyDeviceToHost));
-  }
   return MBT_OK;
 }
 
