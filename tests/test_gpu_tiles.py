@@ -65,3 +65,35 @@ def test_tile_edge_bunches(tmp_path, np_):
     assert worst <= RTOL and worst2 <= RTOL
     g.close()
     orc.close()
+
+
+def test_ensemble_of_tile_edge_rings(tmp_path):
+    """Three rings of 4,095, 4,097 and 8,193 macroparticles per bunch in one
+    ensemble: each ring's bunches start on tiles after the padded bunches of the
+    rings before it.  Every ring matches the oracle to RTOL, statistics and the
+    coordinates of its first and last bunch read between turns."""
+    cfgs = []
+    for k, np_ in enumerate((4095, 4097, 8193)):
+        d = str(tmp_path / ("r%d" % k))
+        path = write_variant("wakes", d, distribution={"Np": np_, "iseed_LON": 3 + k, "iseed_HOR": 23 + k,
+                                                        "iseed_VER": 43 + k})
+        cfgs.append(m.Config(path, d + "/"))
+    print("case tile-edge ensemble", flush=True)
+    ens = m.Ensemble(cfgs)
+    ens.prepare()
+    est, _, erf = ens.track(3)
+    for k, cfg in enumerate(cfgs):
+        orc = o.OracleRing(cfg, o.RNG_PHILOX)
+        orc.prepare()
+        ost, _, orf = orc.track(3)
+        e = stat_errors(est[:, ens.first[k]:ens.first[k + 1]], ost).max()
+        worst = 0.0
+        for ib in (0, orc.Nb - 1):
+            cg_, co = ens.get_bunch(ens.first[k] + ib), orc.get_bunch(ib)
+            scale = np.maximum(np.max(np.abs(co), axis=1, keepdims=True), 1e-300)
+            worst = max(worst, float(np.max(np.abs(cg_ - co) / scale)))
+        print("ring %d (Np %d): stats %.2g, coordinates %.2g" % (k, orc.Np, e, worst), flush=True)
+        assert e <= RTOL and worst <= RTOL
+        assert (np.abs(erf[:, k] - orf) / np.abs(orf)).max() <= RTOL
+        orc.close()
+    ens.close()
