@@ -6,7 +6,9 @@ Tolerances (FP64 everywhere on both sides):
   * centroids: |gpu - oracle| <= RTOL * rms of the same coordinate (centroids of
     a centred bunch are ~1e-3 sigma, so a relative error of the centroid itself is
     not meaningful);
-  * RF phase / voltage: relative error <= RTOL.
+  * RF phase / voltage: relative error <= RTOL;
+  * final coordinates of the first and last bunch: |gpu - oracle| <= RTOL * rms
+    of the same coordinate.
 The residual difference comes from the device's sin/cos (ocml, <= 1 ulp) against
 glibc's, and from the summation order of moments and of the RW sum.  The QE
 normals are bit-identical on both sides (counter-based stream, explicit-FMA
@@ -84,12 +86,15 @@ def test_turns_match_oracle(tmp_path, name):
     fg = g.final_stats()
     assert stat_errors(fg, fo).max() <= RTOL
     # coordinates themselves
+    ce = 0.0
     for ib in (0, g.Nb - 1):
         co = orc.get_bunch(ib)
         cg_ = g.get_bunch(ib)
         sc = np.std(co, axis=1, keepdims=True)
         sc = np.maximum(np.where(sc > 0, sc, np.abs(co).max(axis=1, keepdims=True)), 1e-300)
-        assert (np.abs(cg_ - co) / sc).max() <= 1e-9
+        ce = max(ce, float((np.abs(cg_ - co) / sc).max()))
+    print("%s: coordinate err %.3g" % (name, ce))
+    assert ce <= RTOL
     g.close()
 
 
