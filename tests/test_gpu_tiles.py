@@ -97,3 +97,30 @@ def test_ensemble_of_tile_edge_rings(tmp_path):
         assert (np.abs(erf[:, k] - orf) / np.abs(orf)).max() <= RTOL
         orc.close()
     ens.close()
+
+
+@pytest.mark.parametrize("np_", [4097, 8193])
+def test_quantum_excitation_across_tiles(tmp_path, np_):
+    """Quantum excitation on bunches longer than a tile: each macroparticle's
+    normals come from a counter-based stream keyed by its index in the bunch
+    (not by its slot in the tiles), so they equal the oracle's bit for bit and
+    the run meets RTOL, statistics and coordinates, on every tile of the
+    bunch."""
+    path = write_variant("qe", str(tmp_path), distribution={"Np": np_})
+    cfg = m.Config(path, str(tmp_path) + "/")
+    print("case qe across tiles: Np", np_, flush=True)
+    g = m.Ring(cfg)
+    orc = o.OracleRing(cfg, o.RNG_PHILOX)
+    e0 = stat_errors(g.prepare(), orc.prepare()).max()
+    gst, _, _ = g.track(4)
+    ost, _, _ = orc.track(4)
+    e = stat_errors(gst, ost).max()
+    worst = 0.0
+    for ib in range(g.Nb):
+        cg_, co = g.get_bunch(ib), orc.get_bunch(ib)
+        sc = np.maximum(np.std(co, axis=1, keepdims=True), 1e-300)
+        worst = max(worst, float(np.max(np.abs(cg_ - co) / sc)))
+    print("qe Np %d: stats %.2g (prepare %.2g), coordinates %.2g of the rms" % (np_, e, e0, worst), flush=True)
+    assert e0 <= RTOL and e <= RTOL and worst <= RTOL
+    g.close()
+    orc.close()
