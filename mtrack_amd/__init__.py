@@ -193,10 +193,16 @@ class Ring:
     __del__ = close
 
     def set_bunch(self, ib, soa6):
+        """Coordinates [6, Np] of bunch ib, taken as the bunch at the end of a
+        turn (any resistive-wall kick still pending for it is dropped)."""
         a = np.ascontiguousarray(soa6, dtype=np.float64).reshape(-1)
+        if a.size != 6 * self.Np:
+            raise MbtError("bunch %d holds %d macroparticles" % (ib, self.Np))
         _check(lib().mbt_ring_set_bunch(self.ptr, ib, dptr(a)))
 
     def get_bunch(self, ib):
+        """Coordinates [6, Np] of bunch ib at the end of the last turn (the
+        resistive-wall kick the next pass would apply is included)."""
         a = np.zeros(6 * self.Np)
         _check(lib().mbt_ring_get_bunch(self.ptr, ib, dptr(a)))
         return a.reshape(6, self.Np)

@@ -47,6 +47,8 @@ def test_tile_edge_bunches(tmp_path, np_):
     g.set_bunch(g.Nb - 1, b)
     orc.set_bunch(g.Nb - 1, b)
     assert np.array_equal(g.get_bunch(g.Nb - 1), b)
+    with pytest.raises(m.MbtError):
+        g.set_bunch(0, b[:, :-1])  # one macroparticle short
     e0 = stat_errors(g.prepare(), orc.prepare()).max()
     gst, _, _ = g.track(3)
     ost, _, _ = orc.track(3)
