@@ -105,6 +105,11 @@ def test_rw_gemm_equals_per_bunch_sum(tmp_path):
         orc.prepare()
         ost, _, _ = orc.track(TURNS)
         assert stat_errors(sa[:, a.first[k]:a.first[k + 1]], ost).max() <= RTOL
+        # coordinates read between turns carry the GEMM path's pending kick
+        for ib in (0, orc.Nb - 1):
+            co = orc.get_bunch(ib)
+            sc = np.maximum(np.std(co, axis=1, keepdims=True), 1e-300)
+            assert (np.abs(a.get_bunch(a.first[k] + ib) - co) / sc).max() <= RTOL, (k, ib)
         orc.close()
     a.close()
     b.close()
